@@ -1,0 +1,142 @@
+/*
+ * ihg.h -- C ABI of libihg, the MI355X (gfx950) HIP implementation of
+ * InstaHeat's right-hand side and time-stepping hot path.
+ *
+ * Plain C: device pointers are `double *` into HBM (from ihg_malloc), sizes
+ * are counts, every call returns 0 or a negative IHG_E* code (message from
+ * ihg_last_error()).  The library never exits the process.  All work of a
+ * context is queued on that context's own HIP stream; calls that hand data
+ * to the host synchronise that stream.  A context is not thread safe (as the
+ * reference's globals are not); separate contexts are independent.
+ *
+ * State layout (src/main_template.h:223-233, src/setup.c:249-271):
+ *   y = [phi(N) | dphi(N) | psi(N) | dpsi(N) | h1(Next) | h2(Next) |
+ *        dh1(Next) | dh2(Next) | a(1)],  N = Nx*Ny*Nz, Next = 2*Nx*Ny*(Nz/2+1)
+ * real fields row-major with z fastest, h fields as FFTW r2c half spectra
+ * with interleaved re/im doubles.
+ */
+#ifndef IHG_H
+#define IHG_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IHG_OK 0
+#define IHG_EINVAL (-22)  /* bad argument / unsupported size */
+#define IHG_ENOMEM (-12)  /* device allocation failed */
+#define IHG_EHIP (-1000)  /* a HIP runtime call failed */
+
+/* evo_flags (src/toolbox.h:16-24) as a bit set */
+#define IHG_FILTER 1u
+#define IHG_OUTPUT 2u
+
+/* Runtime form of the compile-time parameters of src/main_template.h:116-162
+ * that the hot path reads.  Grid sizes: powers of two, 4..512 per axis. */
+typedef struct ihg_params {
+    int nx, ny, nz;        /* GRIDPOINTS_X/Y/Z */
+    double xa, xb;         /* SPATIAL_LOWER/UPPER_BOUND_X */
+    double ya, yb;         /* SPATIAL_LOWER/UPPER_BOUND_Y */
+    double za, zb;         /* SPATIAL_LOWER/UPPER_BOUND_Z */
+    double mass;           /* MASS */
+    int enable_gw;         /* ENABLE_GW */
+    int enable_filter;     /* ENABLE_FFT_FILTER */
+    int bins;              /* POWER_SPECTRUM_BINS */
+} ihg_params;
+
+typedef struct ihg_ctx ihg_ctx;
+
+/* Set-up and tear-down of the global state of src/setup.c:89-108
+ * (sizes, k grid, filter mask, FFT tables, work arrays). */
+int ihg_create(const ihg_params *p, ihg_ctx **out);
+int ihg_destroy(ihg_ctx *c);
+/* out = {N, M, Next, Ntot} (src/setup.c:249-271) */
+int ihg_sizes(const ihg_ctx *c, size_t out[4]);
+const char *ihg_last_error(void);
+
+/* Device memory and transfers on the context's stream (replacing
+ * fftw_malloc of the state arrays, src/setup.c:354-477).  ihg_malloc
+ * returns zeroed memory. */
+int ihg_malloc(ihg_ctx *c, double **ptr, size_t count);
+int ihg_free(ihg_ctx *c, double *ptr);
+int ihg_upload(ihg_ctx *c, double *dst, const double *src, size_t count);
+int ihg_download(ihg_ctx *c, double *dst, const double *src, size_t count);
+int ihg_copy(ihg_ctx *c, double *dst, const double *src, size_t count);
+int ihg_zero(ihg_ctx *c, double *dst, size_t count);
+int ihg_sync(ihg_ctx *c);
+
+/* fft / ifft (src/toolbox.h:32-33, src/toolbox.c:806-827): 3D r2c and c2r,
+ * unnormalised, FFTW sign and layout; `out` of r2c and `in` of c2r are
+ * M complex values (2M doubles).  Out of place; c2r keeps its input. */
+int ihg_fft_r2c(ihg_ctx *c, const double *in, double *out);
+int ihg_fft_c2r(ihg_ctx *c, const double *in, double *out);
+
+/* The array mk_summary reads (the reference's global `field`,
+ * src/toolbox.c:638-647); defaults to the `f` of each call. */
+int ihg_set_field(ihg_ctx *c, const double *field);
+
+/* mk_rhs (src/toolbox.h:28, src/toolbox.c:75-88): result = dy/dt at f.
+ * With IHG_FILTER (and enable_filter) phi, dphi, psi, dpsi in f are replaced
+ * by their filtered versions, as in the reference.  With IHG_OUTPUT the
+ * observables of output_all are computed (read with ihg_observables). */
+int ihg_mk_rhs(ihg_ctx *c, double t, double *f, double *result,
+               unsigned flags);
+/* rho_mean, pressure_mean of the last mk_rhs (synchronises) */
+int ihg_means(ihg_ctx *c, double out[2]);
+/* observables of the last IHG_OUTPUT call, packed as
+ * smry[6][4] (phi, dphi, psi, dpsi, rho, pressure: mean, var, min, max),
+ * cstr[4], phi_ps[bins], psi_ps[bins], rho_ps[bins], gw[bins] */
+int ihg_observables(ihg_ctx *c, double *out, size_t len);
+/* mon.calls_rhs (src/toolbox.c:77) */
+unsigned long ihg_calls_rhs(const ihg_ctx *c);
+
+/* mk_initial_psi (src/setup.c:825-903): psi, dpsi from phi, dphi, a */
+int ihg_mk_initial_psi(ihg_ctx *c, double ti, double *field, double *dfield);
+
+/* RK4 stage arithmetic of run_rk4 (src/rk4.c:78-102):
+ *   ihg_rk4_stage: out = y + dt * k / 2 (half = 1) or y + dt * k (half = 0)
+ *   ihg_rk4_final: y += dt * (k1 + 2 k2 + 2 k3 + k4) / 6 */
+int ihg_rk4_stage(ihg_ctx *c, double *out, const double *y, const double *k,
+                  double dt, int half);
+int ihg_rk4_final(ihg_ctx *c, double *y, const double *k1, const double *k2,
+                  const double *k3, const double *k4, double dt);
+
+/* DOPRI853 stage arithmetic of try_step (src/dopri853.c:331-458).
+ * k[0] = dfield (FSAL), k[s-1] = RHS of stage s (s = 1..12).
+ *   ihg_dp853_stage: out = y + dt * sum_j a_{s,j} k[j]  (s = 2..12)
+ *   ihg_dp853_finish: y_new = y + dt * sum_j b_j k[j] and the two sums of
+ *     error() (src/dopri853.c:469-486): sums[0] = sum (yerr2/sk)^2 (5th
+ *     order), sums[1] = sum (yerr/sk)^2 (3rd order).  Synchronises. */
+int ihg_dp853_stage(ihg_ctx *c, int s, double dt, const double *y,
+                    const double *const *k, double *out);
+int ihg_dp853_finish(ihg_ctx *c, double dt, const double *y,
+                     const double *const *k, double *y_new, double rtol,
+                     double atol, double sums[2]);
+/* the tableau the stage kernels use (row-major a[13][12], c[13], b[12],
+ * bhh[3], er[12]) */
+int ihg_dp853_tableau(double *a, double *c, double *b, double *bhh,
+                      double *er);
+
+/* Live timing of the kernel families (the runtime_* monitors of
+ * src/main_template.h:292-311, per kernel): when enabled, every launch is
+ * bracketed by HIP events on the context's stream.  ihg_prof_read returns,
+ * per family, the summed milliseconds, the summed algorithmic bytes (each
+ * input and output array counted once) and the number of launches since the
+ * last reset; it synchronises. */
+int ihg_prof_enable(ihg_ctx *c, int on);
+int ihg_prof_reset(ihg_ctx *c);
+int ihg_prof_read(ihg_ctx *c, double *ms, double *bytes, long *count,
+                  int nfam);
+int ihg_prof_nfam(void);
+const char *ihg_prof_family(int fam);
+
+/* Debug access to intermediate arrays of the last mk_rhs, copied to host:
+ * 0..2 d_x/y/z phi (GW on), 3 rho, 4 pressure, 5 tmp.f (output calls) */
+int ihg_aux(ihg_ctx *c, int which, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

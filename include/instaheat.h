@@ -1,0 +1,139 @@
+/*
+ * instaheat.h -- C99 host of the InstaHeat hot path on MI355X.
+ *
+ * One ih_sim holds what the reference keeps in globals (pars, field, dfield,
+ * field_new, dfield_new, evo_flags, mon; src/main_template.h:313-355) for
+ * one parameter set, with the state arrays resident in HBM.  The steppers
+ * follow run_rk4 (src/rk4.c:27-132) and run_dopri853 with perform_step /
+ * try_step / error / success (src/dopri853.c:112-540) step for step; the
+ * arithmetic runs in libihg (include/ihg.h).
+ *
+ * instaheat_compat.h binds the reference's own names (mk_rhs, run_rk4,
+ * run_dopri853, fft, ifft, evo_flags) to one global ih_sim.
+ */
+#ifndef INSTAHEAT_H
+#define INSTAHEAT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "ihg.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* INTEGRATION_METHOD values (src/main_template.h:51-56) */
+#define IH_RK4 0
+#define IH_DOPRI853 4
+
+/* evo_flags (src/toolbox.h:16-24) */
+struct ih_evolution_flags {
+    uint8_t filter;
+    uint8_t output;
+};
+
+/* the parameters of src/parameters.sh that the hot path reads */
+typedef struct ih_config {
+    ihg_params grid;          /* GRIDPOINTS_*, SPATIAL_*, MASS, ENABLE_GW,
+                                 ENABLE_FFT_FILTER, POWER_SPECTRUM_BINS */
+    int integration_method;   /* INTEGRATION_METHOD (IH_RK4, IH_DOPRI853) */
+    double ti, tf, dt;        /* INITIAL_TIME, FINAL_TIME, DELTA_T */
+    double dt_min;            /* MINIMAL_DELTA_T */
+    long max_steps;           /* MAX_STEPS */
+    double beta, safe;        /* BETA, SAFE */
+    double minscale, maxscale;/* SMALLEST_SCALING, LARGEST_SCALING */
+    double rtol, atol;        /* RELATIVE_TOLERANCE, ABSOLUTE_TOLERANCE */
+    double max_dt_hubble_fraction; /* MAX_DT_HUBBLE_FRACTION (<= 0: off) */
+    long skip;                /* TIME_STEP_SKIPS */
+    double a_initial;         /* A_INITIAL */
+} ih_config;
+
+/* parameters.sh defaults (src/parameters.sh) for the given grid */
+void ih_config_defaults(ih_config *cfg);
+
+typedef struct ih_sim ih_sim;
+
+/* save() hook (src/io.c:494-613): time, scale factor and the packed
+ * observables of include/ihg.h's ihg_observables */
+typedef void (*ih_save_fn)(void *user, double t, double a, const double *obs,
+                           size_t nobs);
+
+/* allocate_and_init_all (src/setup.c:89-108) without the initial data */
+int ih_create(const ih_config *cfg, ih_sim **out);
+void ih_destroy(ih_sim *s);
+ihg_ctx *ih_gpu(ih_sim *s);
+const ih_config *ih_cfg(const ih_sim *s);
+/* device pointers of field / dfield (Ntot doubles each) */
+double *ih_field(ih_sim *s);
+double *ih_dfield(ih_sim *s);
+struct ih_evolution_flags *ih_flags(ih_sim *s);
+double ih_time(const ih_sim *s);
+void ih_set_time(ih_sim *s, double t);
+/* pars.t.dt: the first step size of the next run (DELTA_T initially) */
+void ih_set_dt(ih_sim *s, double dt);
+/* MAX_STEPS of the next run */
+void ih_set_max_steps(ih_sim *s, long max_steps);
+
+/* mk_rhs(t, f, result) with the flags in ih_flags(s) */
+int ih_mk_rhs(ih_sim *s, double t, double *f, double *result);
+
+/* Initial data: a seeded Gaussian random field delta phi with
+ * |delta phi_k|^2 = amplitude^2 / (2 a^3 omega_k), omega_k^2 = k^2 + m^2,
+ * for 0 < |k| <= k_cut (counter-based RNG, Box-Muller, uniform phases);
+ * phi = phi0 + delta phi, dphi = dphi0, a = a0, psi = dpsi = h = 0. */
+int ih_init_gaussian(ih_sim *s, uint64_t seed, double amplitude, double phi0,
+                     double dphi0, double k_cut);
+/* mk_initial_psi (src/setup.c:825-903) on the current field */
+int ih_mk_initial_psi(ih_sim *s);
+
+/* run_rk4 (src/rk4.c:27-132); returns the number of steps or < 0 */
+long ih_run_rk4(ih_sim *s, ih_save_fn save, void *user);
+
+/* one record per attempted DOPRI853 step */
+typedef struct ih_dp_record {
+    double t, dt, err, rho_mean;
+    int accepted;
+} ih_dp_record;
+
+typedef struct ih_dp_stats {
+    long n_stp, n_ok, n_bad, attempts;
+    double t, dt_next, err_old;
+    double integration_seconds; /* mon.integration (src/dopri853.c:117,156) */
+    unsigned long calls_rhs;
+} ih_dp_stats;
+
+/* Optional step proposer consulted before success() on accepted steps:
+ * given the classic controller's dt_next it may return another positive
+ * proposal (the embedded error still judges every step). */
+typedef double (*ih_dt_proposer)(void *user, const ih_dp_record *history,
+                                 long n_history, double dt_classic);
+
+typedef struct ih_dp_opts {
+    const double *forced_dt;  /* non-NULL: take exactly these steps */
+    long n_forced;
+    ih_dp_record *log;        /* may be NULL */
+    long log_cap;
+    ih_dt_proposer proposer;  /* may be NULL */
+    void *proposer_user;
+} ih_dp_opts;
+
+/* run_dopri853 (src/dopri853.c:112-158) */
+long ih_run_dopri853(ih_sim *s, const ih_dp_opts *opts, ih_save_fn save,
+                     void *user, ih_dp_stats *stats);
+
+/* success() (src/dopri853.c:505-540) on a controller state, exported so its
+ * arithmetic can be checked against the oracle's */
+typedef struct ih_ctrl {
+    double err_old, dt_next;
+    int reject;
+} ih_ctrl;
+int ih_success(ih_ctrl *ctl, const ih_config *cfg, double err, double *dt,
+               double rho_mean);
+/* error() (src/dopri853.c:469-486) from the two sums of ihg_dp853_finish */
+double ih_error_norm(double e5, double e3, double dt, size_t ntot);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

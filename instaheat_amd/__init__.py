@@ -1,0 +1,538 @@
+"""instaheat_amd -- InstaHeat's RHS + time-stepping hot path on MI355X.
+
+Python mirror of the two native libraries built in this directory:
+
+* ``libihg.so`` (HIP, gfx950): the C ABI of ``include/ihg.h`` -- fft / ifft,
+  mk_rhs, the observables, mk_initial_psi and the RK4 / DOPRI853 stage
+  arithmetic, on arrays resident in HBM.
+* ``libinstaheat.so`` (C99): the host of ``include/instaheat.h`` -- the
+  reference's run_rk4 / run_dopri853 control flow over libihg.
+
+There is no CPU fallback: importing works without a GPU, but every call that
+needs the device fails loudly when the libraries are missing or no GPU is
+present.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = ["Gpu", "DeviceArray", "Sim", "IhgError", "lib", "host_lib",
+           "LIB_PATH", "HOST_LIB_PATH", "IHG_FILTER", "IHG_OUTPUT",
+           "IH_RK4", "IH_DOPRI853", "obs_len", "unpack_obs", "tableau",
+           "default_config", "IhCtrl", "IhConfig"]
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+# INSTAHEAT_AMD_LIBDIR points at another build of the same two libraries
+LIBDIR = os.environ.get("INSTAHEAT_AMD_LIBDIR", HERE)
+LIB_PATH = os.path.join(LIBDIR, "libihg.so")
+HOST_LIB_PATH = os.path.join(LIBDIR, "libinstaheat.so")
+
+IHG_FILTER = 1
+IHG_OUTPUT = 2
+IH_RK4 = 0
+IH_DOPRI853 = 4
+
+_dptr = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+
+
+class IhgError(RuntimeError):
+    pass
+
+
+class IhgParams(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("xa", ctypes.c_double), ("xb", ctypes.c_double),
+        ("ya", ctypes.c_double), ("yb", ctypes.c_double),
+        ("za", ctypes.c_double), ("zb", ctypes.c_double),
+        ("mass", ctypes.c_double),
+        ("enable_gw", ctypes.c_int), ("enable_filter", ctypes.c_int),
+        ("bins", ctypes.c_int),
+    ]
+
+
+class IhConfig(ctypes.Structure):
+    _fields_ = [
+        ("grid", IhgParams),
+        ("integration_method", ctypes.c_int),
+        ("ti", ctypes.c_double), ("tf", ctypes.c_double), ("dt", ctypes.c_double),
+        ("dt_min", ctypes.c_double), ("max_steps", ctypes.c_long),
+        ("beta", ctypes.c_double), ("safe", ctypes.c_double),
+        ("minscale", ctypes.c_double), ("maxscale", ctypes.c_double),
+        ("rtol", ctypes.c_double), ("atol", ctypes.c_double),
+        ("max_dt_hubble_fraction", ctypes.c_double),
+        ("skip", ctypes.c_long), ("a_initial", ctypes.c_double),
+    ]
+
+
+class IhDpRecord(ctypes.Structure):
+    _fields_ = [("t", ctypes.c_double), ("dt", ctypes.c_double),
+                ("err", ctypes.c_double), ("rho_mean", ctypes.c_double),
+                ("accepted", ctypes.c_int)]
+
+
+class IhDpStats(ctypes.Structure):
+    _fields_ = [("n_stp", ctypes.c_long), ("n_ok", ctypes.c_long),
+                ("n_bad", ctypes.c_long), ("attempts", ctypes.c_long),
+                ("t", ctypes.c_double), ("dt_next", ctypes.c_double),
+                ("err_old", ctypes.c_double),
+                ("integration_seconds", ctypes.c_double),
+                ("calls_rhs", ctypes.c_ulong)]
+
+
+PROPOSER_FN = ctypes.CFUNCTYPE(ctypes.c_double, _vp,
+                               ctypes.POINTER(IhDpRecord), ctypes.c_long,
+                               ctypes.c_double)
+
+
+class IhDpOpts(ctypes.Structure):
+    _fields_ = [("forced_dt", _dptr), ("n_forced", ctypes.c_long),
+                ("log", ctypes.POINTER(IhDpRecord)), ("log_cap", ctypes.c_long),
+                ("proposer", PROPOSER_FN), ("proposer_user", _vp)]
+
+
+class IhCtrl(ctypes.Structure):
+    _fields_ = [("err_old", ctypes.c_double), ("dt_next", ctypes.c_double),
+                ("reject", ctypes.c_int)]
+
+
+SAVE_FN = ctypes.CFUNCTYPE(None, _vp, ctypes.c_double, ctypes.c_double, _dptr,
+                           ctypes.c_size_t)
+
+# every symbol declared in include/ihg.h and include/instaheat.h
+IHG_SYMBOLS = [
+    "ihg_create", "ihg_destroy", "ihg_sizes", "ihg_last_error", "ihg_malloc",
+    "ihg_free", "ihg_upload", "ihg_download", "ihg_copy", "ihg_zero",
+    "ihg_sync", "ihg_fft_r2c", "ihg_fft_c2r", "ihg_set_field", "ihg_mk_rhs",
+    "ihg_means", "ihg_observables", "ihg_calls_rhs", "ihg_mk_initial_psi",
+    "ihg_rk4_stage", "ihg_rk4_final", "ihg_dp853_stage", "ihg_dp853_finish",
+    "ihg_dp853_tableau", "ihg_aux", "ihg_prof_enable", "ihg_prof_reset",
+    "ihg_prof_read", "ihg_prof_nfam", "ihg_prof_family",
+]
+HOST_SYMBOLS = [
+    "ih_config_defaults", "ih_create", "ih_destroy", "ih_gpu", "ih_cfg",
+    "ih_field", "ih_dfield", "ih_flags", "ih_time", "ih_set_time", "ih_set_dt",
+    "ih_set_max_steps",
+    "ih_mk_rhs", "ih_init_gaussian", "ih_mk_initial_psi", "ih_run_rk4",
+    "ih_run_dopri853", "ih_success", "ih_error_norm",
+]
+
+_lib = None
+_host = None
+
+
+def lib():
+    """Load libihg.so (the HIP library); raise if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise IhgError(f"{LIB_PATH} is missing: build it with "
+                       "`python -m instaheat_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    i, d, sz = ctypes.c_int, ctypes.c_double, ctypes.c_size_t
+    sig = {
+        "ihg_create": (i, [ctypes.POINTER(IhgParams), ctypes.POINTER(_vp)]),
+        "ihg_destroy": (i, [_vp]),
+        "ihg_sizes": (i, [_vp, ctypes.POINTER(sz)]),
+        "ihg_last_error": (ctypes.c_char_p, []),
+        "ihg_malloc": (i, [_vp, ctypes.POINTER(_vp), sz]),
+        "ihg_free": (i, [_vp, _vp]),
+        "ihg_upload": (i, [_vp, _vp, _vp, sz]),
+        "ihg_download": (i, [_vp, _vp, _vp, sz]),
+        "ihg_copy": (i, [_vp, _vp, _vp, sz]),
+        "ihg_zero": (i, [_vp, _vp, sz]),
+        "ihg_sync": (i, [_vp]),
+        "ihg_fft_r2c": (i, [_vp, _vp, _vp]),
+        "ihg_fft_c2r": (i, [_vp, _vp, _vp]),
+        "ihg_set_field": (i, [_vp, _vp]),
+        "ihg_mk_rhs": (i, [_vp, d, _vp, _vp, ctypes.c_uint]),
+        "ihg_means": (i, [_vp, _dptr]),
+        "ihg_observables": (i, [_vp, _dptr, sz]),
+        "ihg_calls_rhs": (ctypes.c_ulong, [_vp]),
+        "ihg_mk_initial_psi": (i, [_vp, d, _vp, _vp]),
+        "ihg_rk4_stage": (i, [_vp, _vp, _vp, _vp, d, i]),
+        "ihg_rk4_final": (i, [_vp, _vp, _vp, _vp, _vp, _vp, d]),
+        "ihg_dp853_stage": (i, [_vp, i, d, _vp, ctypes.POINTER(_vp), _vp]),
+        "ihg_dp853_finish": (i, [_vp, d, _vp, ctypes.POINTER(_vp), _vp, d, d,
+                                 _dptr]),
+        "ihg_dp853_tableau": (i, [_dptr] * 5),
+        "ihg_aux": (i, [_vp, i, _dptr]),
+        "ihg_prof_enable": (i, [_vp, i]),
+        "ihg_prof_reset": (i, [_vp]),
+        "ihg_prof_read": (i, [_vp, _dptr, _dptr, ctypes.POINTER(ctypes.c_long), i]),
+        "ihg_prof_nfam": (i, []),
+        "ihg_prof_family": (ctypes.c_char_p, [i]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def host_lib():
+    """Load libinstaheat.so (the C99 host); raise if it is not built."""
+    global _host
+    if _host is not None:
+        return _host
+    lib()
+    if not os.path.exists(HOST_LIB_PATH):
+        raise IhgError(f"{HOST_LIB_PATH} is missing: build it with "
+                       "`python -m instaheat_amd.build`")
+    H = ctypes.CDLL(HOST_LIB_PATH)
+    i, d = ctypes.c_int, ctypes.c_double
+    sig = {
+        "ih_config_defaults": (None, [ctypes.POINTER(IhConfig)]),
+        "ih_create": (i, [ctypes.POINTER(IhConfig), ctypes.POINTER(_vp)]),
+        "ih_destroy": (None, [_vp]),
+        "ih_gpu": (_vp, [_vp]),
+        "ih_field": (_vp, [_vp]),
+        "ih_dfield": (_vp, [_vp]),
+        "ih_time": (d, [_vp]),
+        "ih_set_time": (None, [_vp, d]),
+        "ih_set_dt": (None, [_vp, d]),
+        "ih_set_max_steps": (None, [_vp, ctypes.c_long]),
+        "ih_cfg": (_vp, [_vp]),
+        "ih_flags": (_vp, [_vp]),
+        "ih_mk_rhs": (i, [_vp, d, _vp, _vp]),
+        "ih_init_gaussian": (i, [_vp, ctypes.c_uint64, d, d, d, d]),
+        "ih_mk_initial_psi": (i, [_vp]),
+        "ih_run_rk4": (ctypes.c_long, [_vp, SAVE_FN, _vp]),
+        "ih_run_dopri853": (ctypes.c_long, [_vp, ctypes.POINTER(IhDpOpts),
+                                            SAVE_FN, _vp,
+                                            ctypes.POINTER(IhDpStats)]),
+        "ih_success": (i, [ctypes.POINTER(IhCtrl), ctypes.POINTER(IhConfig), d,
+                           _dptr, d]),
+        "ih_error_norm": (d, [d, d, d, ctypes.c_size_t]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(H, name)
+        fn.restype = res
+        fn.argtypes = args
+    _host = H
+    return H
+
+
+def _check(rc):
+    if rc != 0:
+        msg = lib().ihg_last_error().decode(errors="replace")
+        raise IhgError(f"ihg error {rc}: {msg}")
+    return rc
+
+
+def obs_len(bins):
+    return 28 + 4 * bins
+
+
+def unpack_obs(obs, bins):
+    """Name the parts of a packed observable vector (include/ihg.h)."""
+    obs = np.asarray(obs)
+    smry = obs[:24].reshape(6, 4)
+    return {
+        "phi_smry": smry[0], "dphi_smry": smry[1], "psi_smry": smry[2],
+        "dpsi_smry": smry[3], "rho_smry": smry[4], "pressure_smry": smry[5],
+        "constraints": obs[24:28],
+        "phi_ps": obs[28:28 + bins], "psi_ps": obs[28 + bins:28 + 2 * bins],
+        "rho_ps": obs[28 + 2 * bins:28 + 3 * bins],
+        "gw": obs[28 + 3 * bins:28 + 4 * bins],
+    }
+
+
+def make_params(shape, bounds=((0.0, 1.0),) * 3, mass=1.0, gw=False,
+                filt=True, bins=50):
+    nx, ny, nz = shape
+    return IhgParams(nx, ny, nz, bounds[0][0], bounds[0][1], bounds[1][0],
+                     bounds[1][1], bounds[2][0], bounds[2][1], mass, int(gw),
+                     int(filt), bins)
+
+
+class DeviceArray:
+    """A block of doubles in HBM owned by a Gpu context."""
+
+    def __init__(self, gpu, n, ptr=None, owned=True):
+        self.gpu = gpu
+        self.n = int(n)
+        self.owned = owned
+        if ptr is None:
+            p = _vp()
+            _check(lib().ihg_malloc(gpu.ctx, ctypes.byref(p), self.n))
+            ptr = p.value
+        self.ptr = ptr
+
+    def __del__(self):
+        if self.owned and getattr(self, "ptr", None) and self.gpu.ctx:
+            lib().ihg_free(self.gpu.ctx, self.ptr)
+            self.ptr = None
+
+    def offset(self, k):
+        return DeviceArray(self.gpu, self.n - k, self.ptr + 8 * k, owned=False)
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        assert a.size <= self.n
+        _check(lib().ihg_upload(self.gpu.ctx, self.ptr,
+                                a.ctypes.data_as(_vp), a.size))
+
+    def download(self, n=None):
+        n = self.n if n is None else n
+        out = np.empty(n)
+        _check(lib().ihg_download(self.gpu.ctx, out.ctypes.data_as(_vp),
+                                  self.ptr, n))
+        return out
+
+
+class Gpu:
+    """One libihg context: the reference's globals for one parameter set."""
+
+    def __init__(self, shape, bounds=((0.0, 1.0),) * 3, mass=1.0, gw=False,
+                 filt=True, bins=50):
+        self.params = make_params(shape, bounds, mass, gw, filt, bins)
+        self.bins = bins
+        c = _vp()
+        _check(lib().ihg_create(ctypes.byref(self.params), ctypes.byref(c)))
+        self.ctx = c.value
+        s = (ctypes.c_size_t * 4)()
+        _check(lib().ihg_sizes(self.ctx, s))
+        self.N, self.M, self.Next, self.Ntot = [int(v) for v in s]
+        self.shape = tuple(shape)
+        self.cshape = (shape[0], shape[1], shape[2] // 2 + 1)
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            lib().ihg_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        self.close()
+
+    def alloc(self, n):
+        return DeviceArray(self, n)
+
+    def to_device(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        d = DeviceArray(self, a.size)
+        d.upload(a)
+        return d
+
+    def sync(self):
+        _check(lib().ihg_sync(self.ctx))
+
+    def rfftn(self, x):
+        d_in = self.to_device(x)
+        d_out = self.alloc(2 * self.M)
+        _check(lib().ihg_fft_r2c(self.ctx, d_in.ptr, d_out.ptr))
+        return d_out.download().view(np.complex128).reshape(self.cshape)
+
+    def irfftn(self, X):
+        Xd = np.ascontiguousarray(X, dtype=np.complex128).reshape(-1).view(np.float64)
+        d_in = self.to_device(Xd)
+        d_out = self.alloc(self.N)
+        _check(lib().ihg_fft_c2r(self.ctx, d_in.ptr, d_out.ptr))
+        return d_out.download().reshape(self.shape)
+
+    def set_field(self, d):
+        _check(lib().ihg_set_field(self.ctx, d.ptr if d is not None else None))
+
+    def mk_rhs(self, d_f, d_res, t=0.0, flags=0):
+        _check(lib().ihg_mk_rhs(self.ctx, t, d_f.ptr, d_res.ptr, flags))
+
+    def means(self):
+        out = np.zeros(2)
+        _check(lib().ihg_means(self.ctx, out.ctypes.data_as(_dptr)))
+        return out
+
+    def observables(self):
+        out = np.zeros(obs_len(self.bins))
+        _check(lib().ihg_observables(self.ctx, out.ctypes.data_as(_dptr),
+                                     out.size))
+        return out
+
+    @property
+    def calls_rhs(self):
+        return int(lib().ihg_calls_rhs(self.ctx))
+
+    def mk_initial_psi(self, d_field, d_dfield, ti=0.0):
+        _check(lib().ihg_mk_initial_psi(self.ctx, ti, d_field.ptr, d_dfield.ptr))
+
+    def rk4_stage(self, out, y, k, dt, half):
+        _check(lib().ihg_rk4_stage(self.ctx, out.ptr, y.ptr, k.ptr, dt, int(half)))
+
+    def rk4_final(self, y, k1, k2, k3, k4, dt):
+        _check(lib().ihg_rk4_final(self.ctx, y.ptr, k1.ptr, k2.ptr, k3.ptr,
+                                   k4.ptr, dt))
+
+    def dp853_stage(self, s, dt, y, ks, out):
+        arr = (_vp * 12)(*[k.ptr if k is not None else None for k in ks])
+        _check(lib().ihg_dp853_stage(self.ctx, s, dt, y.ptr, arr, out.ptr))
+
+    def dp853_finish(self, dt, y, ks, ynew, rtol, atol):
+        arr = (_vp * 12)(*[k.ptr if k is not None else None for k in ks])
+        sums = np.zeros(2)
+        _check(lib().ihg_dp853_finish(self.ctx, dt, y.ptr, arr, ynew.ptr, rtol,
+                                      atol, sums.ctypes.data_as(_dptr)))
+        return sums
+
+    def aux(self, which):
+        out = np.zeros(self.N)
+        _check(lib().ihg_aux(self.ctx, which, out.ctypes.data_as(_dptr)))
+        return out
+
+
+def prof_enable(ctx, on=True):
+    _check(lib().ihg_prof_enable(ctx, int(on)))
+
+
+def prof_reset(ctx):
+    _check(lib().ihg_prof_reset(ctx))
+
+
+def prof_read(ctx):
+    """{family: (milliseconds, algorithmic bytes, launches)} since reset."""
+    n = lib().ihg_prof_nfam()
+    ms = np.zeros(n)
+    by = np.zeros(n)
+    cnt = np.zeros(n, dtype=np.int64)
+    _check(lib().ihg_prof_read(ctx, ms.ctypes.data_as(_dptr),
+                               by.ctypes.data_as(_dptr),
+                               cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_long)),
+                               n))
+    return {lib().ihg_prof_family(f).decode(): (float(ms[f]), float(by[f]),
+                                                 int(cnt[f]))
+            for f in range(n)}
+
+
+def tableau():
+    a = np.zeros(13 * 12)
+    c = np.zeros(13)
+    b = np.zeros(12)
+    bhh = np.zeros(3)
+    er = np.zeros(12)
+    _check(lib().ihg_dp853_tableau(*[x.ctypes.data_as(_dptr)
+                                     for x in (a, c, b, bhh, er)]))
+    return {"a": a.reshape(13, 12), "c": c, "b": b, "bhh": bhh, "er": er}
+
+
+def default_config():
+    cfg = IhConfig()
+    host_lib().ih_config_defaults(ctypes.byref(cfg))
+    return cfg
+
+
+class Sim:
+    """The C99 host (include/instaheat.h): one simulation on the GPU."""
+
+    def __init__(self, shape, bounds=((0.0, 1.0),) * 3, mass=1.0, gw=False,
+                 filt=True, bins=50, method=IH_DOPRI853, ti=0.0, tf=1.0,
+                 dt=1e-7, dt_min=1e-9, max_steps=10**15, rtol=1e-10,
+                 atol=1e-14, hubble_frac=1e-2, skip=1, a_initial=1.0,
+                 beta=0.0, safe=0.9, minscale=0.333, maxscale=3.0):
+        cfg = default_config()
+        cfg.grid = make_params(shape, bounds, mass, gw, filt, bins)
+        cfg.integration_method = method
+        cfg.ti, cfg.tf, cfg.dt, cfg.dt_min = ti, tf, dt, dt_min
+        cfg.max_steps = min(int(max_steps), 2**62)
+        cfg.beta, cfg.safe, cfg.minscale, cfg.maxscale = beta, safe, minscale, maxscale
+        cfg.rtol, cfg.atol, cfg.max_dt_hubble_fraction = rtol, atol, hubble_frac
+        cfg.skip, cfg.a_initial = skip, a_initial
+        self.cfg = cfg
+        self.bins = bins
+        s = _vp()
+        _check(host_lib().ih_create(ctypes.byref(cfg), ctypes.byref(s)))
+        self.sim = s.value
+        self.ctx = host_lib().ih_gpu(self.sim)
+        sz = (ctypes.c_size_t * 4)()
+        _check(lib().ihg_sizes(self.ctx, sz))
+        self.N, self.M, self.Next, self.Ntot = [int(v) for v in sz]
+        self.shape = tuple(shape)
+
+    def close(self):
+        if getattr(self, "sim", None):
+            host_lib().ih_destroy(self.sim)
+            self.sim = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def t(self):
+        return host_lib().ih_time(self.sim)
+
+    @t.setter
+    def t(self, v):
+        host_lib().ih_set_time(self.sim, v)
+
+    def set_dt(self, dt):
+        host_lib().ih_set_dt(self.sim, dt)
+        self.cfg.dt = dt
+
+    def set_max_steps(self, n):
+        host_lib().ih_set_max_steps(self.sim, int(n))
+        self.cfg.max_steps = int(n)
+
+    def _dl(self, ptr, n):
+        out = np.empty(n)
+        _check(lib().ihg_download(self.ctx, out.ctypes.data_as(_vp), ptr, n))
+        return out
+
+    def field(self):
+        return self._dl(host_lib().ih_field(self.sim), self.Ntot)
+
+    def dfield(self):
+        return self._dl(host_lib().ih_dfield(self.sim), self.Ntot)
+
+    def set_field(self, y):
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        assert y.size == self.Ntot
+        _check(lib().ihg_upload(self.ctx, host_lib().ih_field(self.sim),
+                                y.ctypes.data_as(_vp), self.Ntot))
+
+    def init_gaussian(self, seed=1, amplitude=1e-5, phi0=1.0093, dphi0=-0.7137,
+                      k_cut=None):
+        if k_cut is None:
+            k_cut = self.shape[0] * (2 * np.pi / (self.cfg.grid.xb - self.cfg.grid.xa)) / 4
+        _check(host_lib().ih_init_gaussian(self.sim, seed, amplitude, phi0,
+                                           dphi0, k_cut))
+
+    def mk_initial_psi(self):
+        _check(host_lib().ih_mk_initial_psi(self.sim))
+
+    def run_rk4(self, saves=None):
+        def cb(user, t, a, obs, n):
+            if saves is not None:
+                saves.append((t, a, np.ctypeslib.as_array(obs, (n,)).copy()))
+        fn = SAVE_FN(cb)
+        n = host_lib().ih_run_rk4(self.sim, fn, None)
+        if n < 0:
+            _check(n)
+        return n
+
+    def run_dopri853(self, forced=None, saves=None, log_cap=0, proposer=None):
+        fz = None
+        if forced is not None:
+            fz = np.ascontiguousarray(forced, dtype=np.float64)
+        log = (IhDpRecord * log_cap)() if log_cap else None
+        prop = PROPOSER_FN(proposer) if proposer is not None else PROPOSER_FN()
+        opts = IhDpOpts(fz.ctypes.data_as(_dptr) if fz is not None else None,
+                        len(fz) if fz is not None else 0,
+                        ctypes.cast(log, ctypes.POINTER(IhDpRecord)) if log else None,
+                        log_cap, prop, None)
+        st = IhDpStats()
+
+        def cb(user, t, a, obs, n):
+            if saves is not None:
+                saves.append((t, a, np.ctypeslib.as_array(obs, (n,)).copy()))
+        fn = SAVE_FN(cb)
+        n = host_lib().ih_run_dopri853(self.sim, ctypes.byref(opts), fn, None,
+                                       ctypes.byref(st))
+        if n < 0:
+            _check(n)
+        out = {k: getattr(st, k) for k, _ in IhDpStats._fields_}
+        if log:
+            m = min(n, log_cap)
+            out["log"] = [(log[i].t, log[i].dt, log[i].err, log[i].rho_mean,
+                           bool(log[i].accepted)) for i in range(m)]
+        return out

@@ -1,0 +1,260 @@
+// fft_core.h -- FP64 complex FFT building blocks for gfx950.
+//
+// A line of L complex points is transformed by a work-group slice of T = L/E
+// threads.  Each stage is a radix-R Stockham pass: a thread loads E values
+// (E/R butterflies), applies twiddles, does the R-point DFTs in registers and
+// writes the results back to LDS in Stockham order, so that after the last
+// stage the line sits in LDS in natural order.  The first stage can read its
+// inputs through a caller-supplied functor (straight from HBM for the
+// contiguous z axis), every later stage reads LDS.
+//
+// Sign convention: DIR = -1 is the forward transform (e^{-i}), DIR = +1 the
+// backward one; both unnormalised, matching FFTW's r2c/c2r.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace ihg {
+
+__device__ __forceinline__ double2 cadd(double2 a, double2 b)
+{
+    return make_double2(a.x + b.x, a.y + b.y);
+}
+__device__ __forceinline__ double2 csub(double2 a, double2 b)
+{
+    return make_double2(a.x - b.x, a.y - b.y);
+}
+__device__ __forceinline__ double2 cmul(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cconj(double2 a)
+{
+    return make_double2(a.x, -a.y);
+}
+__device__ __forceinline__ double2 cscale(double2 a, double s)
+{
+    return make_double2(a.x * s, a.y * s);
+}
+// a * (DIR * i)
+template <int DIR>
+__device__ __forceinline__ double2 mul_i(double2 a)
+{
+    if constexpr (DIR < 0)
+        return make_double2(a.y, -a.x);
+    else
+        return make_double2(-a.y, a.x);
+}
+
+// a * e^{DIR * 2 pi i m / 16} for a compile-time m
+template <int DIR, int M>
+__device__ __forceinline__ double2 rot16(double2 a)
+{
+    constexpr int m = M & 15;
+    constexpr double C1 = 0.92387953251128675613; // cos(pi/8)
+    constexpr double S1 = 0.38268343236508977173; // sin(pi/8)
+    constexpr double H = 0.70710678118654752440;  // cos(pi/4)
+    if constexpr (m == 0) {
+        return a;
+    } else if constexpr (m == 4) {
+        return mul_i<DIR>(a);
+    } else if constexpr (m == 8) {
+        return make_double2(-a.x, -a.y);
+    } else if constexpr (m == 12) {
+        return mul_i<-DIR>(a);
+    } else {
+        // cos / sin of 2 pi m / 16 for m in the four quadrants
+        constexpr double cs[16] = {1, C1, H, S1, 0, -S1, -H, -C1,
+                                   -1, -C1, -H, -S1, 0, S1, H, C1};
+        constexpr double sn[16] = {0, S1, H, C1, 1, C1, H, S1,
+                                   0, -S1, -H, -C1, -1, -C1, -H, -S1};
+        constexpr double c = cs[m];
+        constexpr double s = DIR * sn[m];
+        return make_double2(a.x * c - a.y * s, a.x * s + a.y * c);
+    }
+}
+
+template <int DIR>
+__device__ __forceinline__ void dft2(double2 *v)
+{
+    double2 t = v[0];
+    v[0] = cadd(t, v[1]);
+    v[1] = csub(t, v[1]);
+}
+
+template <int DIR>
+__device__ __forceinline__ void dft4(double2 &x0, double2 &x1, double2 &x2,
+                                     double2 &x3)
+{
+    double2 a0 = cadd(x0, x2), a1 = csub(x0, x2);
+    double2 b0 = cadd(x1, x3), b1 = mul_i<DIR>(csub(x1, x3));
+    x0 = cadd(a0, b0);
+    x2 = csub(a0, b0);
+    x1 = cadd(a1, b1);
+    x3 = csub(a1, b1);
+}
+
+template <int DIR>
+__device__ __forceinline__ void dft4(double2 *v)
+{
+    dft4<DIR>(v[0], v[1], v[2], v[3]);
+}
+
+template <int DIR>
+__device__ __forceinline__ void dft8(double2 *v)
+{
+    double2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+    double2 o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    dft4<DIR>(e0, e1, e2, e3);
+    dft4<DIR>(o0, o1, o2, o3);
+    o1 = rot16<DIR, 2>(o1);
+    o2 = rot16<DIR, 4>(o2);
+    o3 = rot16<DIR, 6>(o3);
+    v[0] = cadd(e0, o0);
+    v[4] = csub(e0, o0);
+    v[1] = cadd(e1, o1);
+    v[5] = csub(e1, o1);
+    v[2] = cadd(e2, o2);
+    v[6] = csub(e2, o2);
+    v[3] = cadd(e3, o3);
+    v[7] = csub(e3, o3);
+}
+
+// 16 = 4 x 4: n = 4 p + q, k = k1 + 4 k2
+template <int DIR>
+__device__ __forceinline__ void dft16(double2 *v)
+{
+    double2 y[4][4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        y[q][0] = v[q];
+        y[q][1] = v[4 + q];
+        y[q][2] = v[8 + q];
+        y[q][3] = v[12 + q];
+        dft4<DIR>(y[q][0], y[q][1], y[q][2], y[q][3]);
+    }
+    y[1][1] = rot16<DIR, 1>(y[1][1]);
+    y[1][2] = rot16<DIR, 2>(y[1][2]);
+    y[1][3] = rot16<DIR, 3>(y[1][3]);
+    y[2][1] = rot16<DIR, 2>(y[2][1]);
+    y[2][2] = rot16<DIR, 4>(y[2][2]);
+    y[2][3] = rot16<DIR, 6>(y[2][3]);
+    y[3][1] = rot16<DIR, 3>(y[3][1]);
+    y[3][2] = rot16<DIR, 6>(y[3][2]);
+    y[3][3] = rot16<DIR, 9>(y[3][3]);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+        dft4<DIR>(y[0][k1], y[1][k1], y[2][k1], y[3][k1]);
+        v[k1] = y[0][k1];
+        v[k1 + 4] = y[1][k1];
+        v[k1 + 8] = y[2][k1];
+        v[k1 + 12] = y[3][k1];
+    }
+}
+
+template <int R, int DIR>
+__device__ __forceinline__ void dft(double2 *v)
+{
+    if constexpr (R == 2)
+        dft2<DIR>(v);
+    else if constexpr (R == 4)
+        dft4<DIR>(v);
+    else if constexpr (R == 8)
+        dft8<DIR>(v);
+    else
+        dft16<DIR>(v);
+}
+
+// Factorisation of a line length: E values per thread, radices R0, R1, R2
+// (R1 / R2 = 1 when absent).  Every radix divides E.
+template <int L>
+struct Plan;
+template <> struct Plan<4>   { static constexpr int E = 4,  R0 = 4,  R1 = 1,  R2 = 1; };
+template <> struct Plan<8>   { static constexpr int E = 8,  R0 = 8,  R1 = 1,  R2 = 1; };
+template <> struct Plan<16>  { static constexpr int E = 16, R0 = 16, R1 = 1,  R2 = 1; };
+template <> struct Plan<32>  { static constexpr int E = 8,  R0 = 8,  R1 = 4,  R2 = 1; };
+template <> struct Plan<64>  { static constexpr int E = 8,  R0 = 8,  R1 = 8,  R2 = 1; };
+template <> struct Plan<128> { static constexpr int E = 16, R0 = 16, R1 = 8,  R2 = 1; };
+template <> struct Plan<256> { static constexpr int E = 16, R0 = 16, R1 = 16, R2 = 1; };
+template <> struct Plan<512> { static constexpr int E = 16, R0 = 16, R1 = 16, R2 = 2; };
+
+// LDS geometry of a tile of B lines: one pad slot every 16 points and an odd
+// line pitch, so that both row-wise and column-wise sweeps spread over banks.
+template <int L>
+struct Lds {
+    static constexpr int PITCH = L + L / 16 + 1;
+    __device__ static __forceinline__ int idx(int n) { return n + (n >> 4); }
+};
+
+// One Stockham stage of radix R on a line in LDS.  Ns = product of the
+// radices of the earlier stages.  `rd` supplies inputs when FIRST (it is
+// called with the point index n), otherwise inputs come from LDS.
+template <int L, int R, int Ns, int DIR, bool FIRST, class Rd>
+__device__ __forceinline__ void stockham_stage(double2 *line, int j,
+                                               const double2 *__restrict__ tw,
+                                               Rd rd)
+{
+    constexpr int E = Plan<L>::E;
+    constexpr int T = L / E;
+    constexpr int NB = E / R; // butterflies per thread
+    constexpr int STRIDE_IN = L / R;
+    double2 v[NB][R];
+#pragma unroll
+    for (int m = 0; m < NB; ++m) {
+        const int jb = j + m * T;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int n = jb + r * STRIDE_IN;
+            if constexpr (FIRST)
+                v[m][r] = rd(n);
+            else
+                v[m][r] = line[Lds<L>::idx(n)];
+        }
+    }
+    if constexpr (!FIRST)
+        __syncthreads();
+#pragma unroll
+    for (int m = 0; m < NB; ++m) {
+        const int jb = j + m * T;
+        if constexpr (Ns > 1) {
+            const int k = jb % Ns;
+            constexpr int TS = L / (Ns * R);
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                double2 w = tw[L + k * r * TS];
+                if constexpr (DIR > 0)
+                    w = cconj(w);
+                v[m][r] = cmul(v[m][r], w);
+            }
+        }
+        dft<R, DIR>(v[m]);
+        const int k = (Ns > 1) ? jb % Ns : 0;
+        const int base = (jb / Ns) * Ns * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            line[Lds<L>::idx(base + r * Ns)] = v[m][r];
+    }
+    __syncthreads();
+}
+
+struct NoRead {
+    __device__ double2 operator()(int) const { return make_double2(0.0, 0.0); }
+};
+
+// Full line transform.  On entry either the line is in LDS (FIRST_FROM_RD =
+// false; callers must have synchronised) or `rd` supplies it.  On exit the
+// transformed line is in LDS in natural order and the block is synchronised.
+template <int L, int DIR, bool FIRST_FROM_RD = false, class Rd = NoRead>
+__device__ __forceinline__ void fft_line(double2 *line, int j,
+                                         const double2 *__restrict__ tw,
+                                         Rd rd = Rd())
+{
+    using P = Plan<L>;
+    stockham_stage<L, P::R0, 1, DIR, FIRST_FROM_RD>(line, j, tw, rd);
+    if constexpr (P::R1 > 1)
+        stockham_stage<L, P::R1, P::R0, DIR, false>(line, j, tw, rd);
+    if constexpr (P::R2 > 1)
+        stockham_stage<L, P::R2, P::R0 * P::R1, DIR, false>(line, j, tw, rd);
+}
+
+} // namespace ihg

@@ -1,0 +1,275 @@
+// fft_kernels.h -- FFT pass kernels for the 3D r2c / c2r transforms.
+//
+// Layout (FFTW r2c_3d, the state layout of the reference): a real field is
+// double[Nx][Ny][Nz] with z fastest; its half spectrum is
+// complex[Nx][Ny][Nz/2+1].  A 3D r2c is a z pass (real rows -> half rows),
+// then a y pass and an x pass on the half spectrum; c2r is the mirror.
+//
+// z passes transform two real rows at once as one complex row (real part one
+// row, imaginary part the other) and split / merge the two Hermitian halves
+// in LDS.  y and x passes are strided: a work-group owns B neighbouring lines,
+// so every global access is a run of B consecutive complex values.
+#pragma once
+
+#include "fft_core.h"
+
+namespace ihg {
+
+constexpr int FFT_WG = 256;
+
+template <int L>
+struct Tile {
+    static constexpr int E = Plan<L>::E;
+    static constexpr int T = L / E;
+    static constexpr int B = FFT_WG / T;
+    static constexpr int PITCH = Lds<L>::PITCH;
+};
+
+// Geometry of a set of strided lines: line l starts at
+// (l / inner) * outer_stride + (l % inner) and its points are `stride` apart.
+struct LineSet {
+    long nlines, inner, outer_stride, stride;
+    __device__ __forceinline__ long base(long l) const
+    {
+        return (l / inner) * outer_stride + (l % inner);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// strided complex-to-complex pass (y or x), in place or out of place,
+// batched over blockIdx.y (arrays `batch` complex values apart)
+// ---------------------------------------------------------------------------
+template <int L, int DIR>
+__global__ __launch_bounds__(FFT_WG) void k_c2c_strided(
+    const double2 *in, double2 *out, LineSet ls, long batch,
+    const double2 *__restrict__ tw)
+{
+    using TL = Tile<L>;
+    __shared__ double2 lds[TL::B * TL::PITCH];
+    const int tid = threadIdx.x;
+    in += blockIdx.y * batch;
+    out += blockIdx.y * batch;
+    const int b = tid % TL::B, n0 = tid / TL::B;
+    const long line = (long)blockIdx.x * TL::B + b;
+    const bool act = line < ls.nlines;
+    const long base = act ? ls.base(line) : 0;
+    double2 *my = lds + b * TL::PITCH;
+#pragma unroll
+    for (int i = 0; i < TL::E; ++i) {
+        const int n = n0 + i * TL::T;
+        my[Lds<L>::idx(n)] =
+            act ? in[base + n * ls.stride] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    fft_line<L, DIR>(lds + (tid / TL::T) * TL::PITCH, tid % TL::T, tw);
+    if (act) {
+#pragma unroll
+        for (int i = 0; i < TL::E; ++i) {
+            const int n = n0 + i * TL::T;
+            out[base + n * ls.stride] = my[Lds<L>::idx(n)];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// x-axis round trip with a spectral factor: forward x transform, multiply by
+// fac(line, kx_index, value), backward x transform; in place, batched.
+// ---------------------------------------------------------------------------
+template <int L, class Fac>
+__global__ __launch_bounds__(FFT_WG) void k_x_roundtrip(
+    double2 *data, LineSet ls, long batch, const double2 *__restrict__ tw,
+    Fac fac)
+{
+    using TL = Tile<L>;
+    __shared__ double2 lds[TL::B * TL::PITCH];
+    const int tid = threadIdx.x;
+    data += blockIdx.y * batch;
+    const int b = tid % TL::B, n0 = tid / TL::B;
+    const long line = (long)blockIdx.x * TL::B + b;
+    const bool act = line < ls.nlines;
+    const long base = act ? ls.base(line) : 0;
+    double2 *my = lds + b * TL::PITCH;
+    double2 *core = lds + (tid / TL::T) * TL::PITCH;
+    const int j = tid % TL::T;
+#pragma unroll
+    for (int i = 0; i < TL::E; ++i) {
+        const int n = n0 + i * TL::T;
+        my[Lds<L>::idx(n)] =
+            act ? data[base + n * ls.stride] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    fft_line<L, -1>(core, j, tw);
+#pragma unroll
+    for (int i = 0; i < TL::E; ++i) {
+        const int n = n0 + i * TL::T;
+        my[Lds<L>::idx(n)] = fac(blockIdx.y, line, n, my[Lds<L>::idx(n)]);
+    }
+    __syncthreads();
+    fft_line<L, +1>(core, j, tw);
+    if (act) {
+#pragma unroll
+        for (int i = 0; i < TL::E; ++i) {
+            const int n = n0 + i * TL::T;
+            data[base + n * ls.stride] = my[Lds<L>::idx(n)];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// x-axis forward transform followed by NQ backward transforms of the same
+// spectrum, each multiplied by its own factor fac(q, line, kx_index, value)
+// (the spectral derivatives of the right-hand side).  Output q goes to
+// out + q * qstride.
+// ---------------------------------------------------------------------------
+template <int L, int NQ, class Fac>
+__global__ __launch_bounds__(FFT_WG) void k_x_fanout(
+    const double2 *in, double2 *out, long qstride, LineSet ls,
+    const double2 *__restrict__ tw, Fac fac)
+{
+    using TL = Tile<L>;
+    __shared__ double2 lds[TL::B * TL::PITCH];
+    const int tid = threadIdx.x;
+    const int b = tid % TL::B, n0 = tid / TL::B;
+    const long line = (long)blockIdx.x * TL::B + b;
+    const bool act = line < ls.nlines;
+    const long base = act ? ls.base(line) : 0;
+    double2 *my = lds + b * TL::PITCH;
+    double2 *core = lds + (tid / TL::T) * TL::PITCH;
+    const int j = tid % TL::T;
+#pragma unroll
+    for (int i = 0; i < TL::E; ++i) {
+        const int n = n0 + i * TL::T;
+        my[Lds<L>::idx(n)] =
+            act ? in[base + n * ls.stride] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    fft_line<L, -1>(core, j, tw);
+    double2 v[TL::E];
+#pragma unroll
+    for (int i = 0; i < TL::E; ++i)
+        v[i] = my[Lds<L>::idx(n0 + i * TL::T)];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+#pragma unroll
+        for (int i = 0; i < TL::E; ++i) {
+            const int n = n0 + i * TL::T;
+            my[Lds<L>::idx(n)] = fac(q, line, n, v[i]);
+        }
+        __syncthreads();
+        fft_line<L, +1>(core, j, tw);
+        if (act) {
+            double2 *o = out + q * qstride;
+#pragma unroll
+            for (int i = 0; i < TL::E; ++i) {
+                const int n = n0 + i * TL::T;
+                o[base + n * ls.stride] = my[Lds<L>::idx(n)];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// z passes
+// ---------------------------------------------------------------------------
+
+// Hermitian split after a forward transform of Z = A + iB (A, B real rows):
+// A[k] = (Z[k] + conj Z[L-k]) / 2, B[k] = -i (Z[k] - conj Z[L-k]) / 2.
+template <int L>
+__device__ __forceinline__ double2 herm_split(const double2 *line, int k,
+                                              int s)
+{
+    const double2 z1 = line[Lds<L>::idx(k)];
+    const double2 z2 = cconj(line[Lds<L>::idx((L - k) & (L - 1))]);
+    if (s == 0)
+        return make_double2(0.5 * (z1.x + z2.x), 0.5 * (z1.y + z2.y));
+    return make_double2(0.5 * (z1.y - z2.y), -0.5 * (z1.x - z2.x));
+}
+
+// Merge of two half rows A, B (Hermitian extension, imaginary parts of the
+// k = 0 and k = L/2 bins dropped) into Z[n] = A_ext[n] + i B_ext[n].
+// Written without branches: the mirrored index and the sign / zeroing of the
+// imaginary parts are selects, so no value the caller computes is left to
+// divergent control flow.
+template <int L>
+__device__ __forceinline__ double2 herm_merge(const double2 *__restrict__ A,
+                                              const double2 *__restrict__ Bv,
+                                              int n)
+{
+    const bool upper = n > L / 2;
+    const int m = upper ? L - n : n;
+    const double2 a = A[m], bb = Bv[m];
+    // imaginary parts of the k = 0 and k = L/2 bins are dropped; the upper
+    // half uses the conjugates
+    const double edge = (m == 0 || m == L / 2) ? 0.0 : 1.0;
+    const double sg = upper ? -1.0 : 1.0;
+    const double ai = sg * edge * a.y, bi = sg * edge * bb.y;
+    return make_double2(a.x - bi, ai + bb.x);
+}
+
+// Real rows -> half rows.  Line l = rows (2l, 2l+1) of `in` (nrows rows of
+// L reals, batched over blockIdx.y with `rbatch` / `cbatch` strides).
+template <int L>
+__global__ __launch_bounds__(FFT_WG) void k_z_r2c(
+    const double *in, double2 *out, long nrows, long rbatch, long cbatch,
+    const double2 *__restrict__ tw)
+{
+    using TL = Tile<L>;
+    constexpr int MZ = L / 2 + 1;
+    __shared__ double2 lds[TL::B * TL::PITCH];
+    const int tid = threadIdx.x;
+    in += blockIdx.y * rbatch;
+    out += blockIdx.y * cbatch;
+    const long nlines = nrows / 2;
+    const int bc = tid / TL::T;
+    const long line = (long)blockIdx.x * TL::B + bc;
+    const bool act = line < nlines;
+    const double *ra = in + (act ? 2 * line : 0) * L;
+    auto rd = [&](int n) {
+        const double u = ra[n], v = ra[L + n]; // in bounds for every lane
+        return act ? make_double2(u, v) : make_double2(0.0, 0.0);
+    };
+    fft_line<L, -1, true>(lds + bc * TL::PITCH, tid % TL::T, tw, rd);
+    const long row0 = 2 * (long)blockIdx.x * TL::B;
+    for (int idx = tid; idx < 2 * TL::B * MZ; idx += FFT_WG) {
+        const int r = idx / MZ, k = idx - r * MZ;
+        if (row0 + r < nrows)
+            out[row0 * MZ + idx] =
+                herm_split<L>(lds + (r >> 1) * TL::PITCH, k, r & 1);
+    }
+}
+
+// Half rows -> real rows (the c2r z pass).
+template <int L>
+__global__ __launch_bounds__(FFT_WG) void k_z_c2r(
+    const double2 *in, double *out, long nrows, long cbatch, long rbatch,
+    const double2 *__restrict__ tw)
+{
+    using TL = Tile<L>;
+    constexpr int MZ = L / 2 + 1;
+    __shared__ double2 lds[TL::B * TL::PITCH];
+    const int tid = threadIdx.x;
+    in += blockIdx.y * cbatch;
+    out += blockIdx.y * rbatch;
+    const long nlines = nrows / 2;
+    const int bc = tid / TL::T;
+    const long line = (long)blockIdx.x * TL::B + bc;
+    const bool act = line < nlines;
+    const double2 *A = in + (act ? 2 * line : 0) * MZ;
+    auto rd = [&](int n) {
+        const double2 z = herm_merge<L>(A, A + MZ, n); // in bounds for every lane
+        return act ? z : make_double2(0.0, 0.0);
+    };
+    fft_line<L, +1, true>(lds + bc * TL::PITCH, tid % TL::T, tw, rd);
+    const long row0 = 2 * (long)blockIdx.x * TL::B;
+    for (int idx = tid; idx < 2 * TL::B * L; idx += FFT_WG) {
+        const int r = idx / L, n = idx - r * L;
+        if (row0 + r < nrows) {
+            const double2 z = lds[(r >> 1) * TL::PITCH + Lds<L>::idx(n)];
+            out[row0 * L + idx] = (r & 1) ? z.y : z.x;
+        }
+    }
+}
+
+} // namespace ihg

@@ -1,0 +1,459 @@
+// ihg_api.hip -- context set-up and the C ABI of libihg (include/ihg.h).
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+#include "ihg_internal.h"
+
+namespace ihg {
+
+static thread_local char g_err[512] = "";
+
+int set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return IHG_EINVAL;
+}
+
+int hip_check(hipError_t e, const char *what)
+{
+    if (e == hipSuccess) return 0;
+    snprintf(g_err, sizeof g_err, "%s: %s", what, hipGetErrorString(e));
+    return (e == hipErrorOutOfMemory) ? IHG_ENOMEM : IHG_EHIP;
+}
+
+static bool pow2_in_range(int n) { return n >= 4 && n <= 512 && !(n & (n - 1)); }
+
+static int check_ptr(const void *p, const char *name)
+{
+    if (!p) return set_error("%s is NULL", name);
+    if (((size_t)p) & 15) return set_error("%s is not 16-byte aligned", name);
+    return 0;
+}
+
+template <class T>
+static int dalloc(T **p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+    if (e != hipSuccess) return hip_check(e, "hipMalloc");
+    return hip_check(hipMemset(*p, 0, count * sizeof(T)), "hipMemset");
+}
+
+// filter_window / mk_filter_mask (src/setup.c:710-754), evaluated on the
+// host with the same libm calls as the reference
+static void host_filter_mask(const ihg_ctx *c, double *out)
+{
+    const int Nx = c->Nx, Ny = c->Ny, Nz = c->Nz, Mz = c->Mz;
+    const KGeom &kg = c->kg;
+    const double kxmax = (Nx / 2) * kg.kx1;
+    const double kymax = (Ny / 2) * kg.ky1;
+    const double kzmax = (Nz / 2) * kg.kz1;
+    for (int i = 0; i < Nx; ++i) {
+        const double kxf = (i > Nx / 2) ? kg.kx1 * (i - Nx) : kg.kx1 * i;
+        for (int j = 0; j < Ny; ++j) {
+            const double kyf = (j > Ny / 2) ? kg.ky1 * (j - Ny) : kg.ky1 * j;
+            for (int k = 0; k < Mz; ++k) {
+                const double kzf = kg.kz1 * k;
+                double frac = fabs(kxf) / kxmax;
+                const double fy = fabs(kyf) / kymax, fz = fabs(kzf) / kzmax;
+                frac = frac > fy ? frac : fy;
+                frac = frac > fz ? frac : fz;
+                out[((long)i * Ny + j) * Mz + k] = exp(-36.0 * pow(frac, 36));
+            }
+        }
+    }
+}
+
+static const char *const FAM_NAMES[F_NFAM] = {
+    "z_r2c", "z_c2r", "y_pass", "x_pass", "x_fanout", "z_c2r_rho", "means",
+    "update_phi_psi", "z_r2c_sources", "tt_update_h", "x_roundtrip",
+    "observables", "dp_stage", "dp_finish", "rk4"};
+
+static int prof_flush(ihg_ctx *c)
+{
+    if (c->prof_pending.empty()) return 0;
+    IHG_TRY(hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize"));
+    for (const ProfRec &r : c->prof_pending) {
+        float ms = 0.0f;
+        IHG_TRY(hip_check(hipEventElapsedTime(&ms, r.a, r.b),
+                          "hipEventElapsedTime"));
+        c->fam_ms[r.fam] += ms;
+        c->fam_bytes[r.fam] += r.bytes;
+        c->fam_count[r.fam] += 1;
+        c->ev_pool.push_back(r.a);
+        c->ev_pool.push_back(r.b);
+    }
+    c->prof_pending.clear();
+    return 0;
+}
+
+static int pool_event(ihg_ctx *c, hipEvent_t *e)
+{
+    if (!c->ev_pool.empty()) {
+        *e = c->ev_pool.back();
+        c->ev_pool.pop_back();
+        return 0;
+    }
+    return hip_check(hipEventCreate(e), "hipEventCreate");
+}
+
+int prof_begin(ihg_ctx *c, int fam, double bytes)
+{
+    if (c->prof_pending.size() >= 65536) IHG_TRY(prof_flush(c));
+    ProfRec r{fam, nullptr, nullptr, bytes};
+    IHG_TRY(pool_event(c, &r.a));
+    IHG_TRY(pool_event(c, &r.b));
+    IHG_TRY(hip_check(hipEventRecord(r.a, c->stream), "hipEventRecord"));
+    c->prof_pending.push_back(r);
+    return 0;
+}
+
+int prof_end(ihg_ctx *c)
+{
+    return hip_check(hipEventRecord(c->prof_pending.back().b, c->stream),
+                     "hipEventRecord");
+}
+
+} // namespace ihg
+
+using namespace ihg;
+
+extern "C" const char *ihg_last_error(void) { return g_err; }
+
+extern "C" int ihg_prof_nfam(void) { return F_NFAM; }
+
+extern "C" const char *ihg_prof_family(int fam)
+{
+    return (fam >= 0 && fam < F_NFAM) ? FAM_NAMES[fam] : "";
+}
+
+extern "C" int ihg_prof_enable(ihg_ctx *c, int on)
+{
+    if (!c) return set_error("NULL context");
+    if (!on) IHG_TRY(prof_flush(c));
+    c->prof_on = on != 0;
+    return 0;
+}
+
+extern "C" int ihg_prof_reset(ihg_ctx *c)
+{
+    if (!c) return set_error("NULL context");
+    IHG_TRY(prof_flush(c));
+    for (int f = 0; f < F_NFAM; ++f) {
+        c->fam_ms[f] = 0.0;
+        c->fam_bytes[f] = 0.0;
+        c->fam_count[f] = 0;
+    }
+    return 0;
+}
+
+extern "C" int ihg_prof_read(ihg_ctx *c, double *ms, double *bytes,
+                             long *count, int nfam)
+{
+    if (!c || !ms || !bytes || !count) return set_error("NULL argument");
+    IHG_TRY(prof_flush(c));
+    for (int f = 0; f < nfam && f < F_NFAM; ++f) {
+        ms[f] = c->fam_ms[f];
+        bytes[f] = c->fam_bytes[f];
+        count[f] = c->fam_count[f];
+    }
+    return 0;
+}
+
+extern "C" int ihg_destroy(ihg_ctx *c)
+{
+    if (!c) return 0;
+    (void)hipStreamSynchronize(c->stream);
+    for (const ProfRec &r : c->prof_pending) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    for (hipEvent_t e : c->ev_pool)
+        (void)hipEventDestroy(e);
+    void *dev[] = {c->tw, c->filter, c->W, c->Wc, c->grad3, c->rho,
+                   c->partials, c->dscal, c->dobs};
+    for (void *p : dev)
+        if (p) (void)hipFree(p);
+    if (c->hscal) (void)hipHostFree(c->hscal);
+    if (c->hobs) (void)hipHostFree(c->hobs);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+extern "C" int ihg_create(const ihg_params *p, ihg_ctx **out)
+{
+    if (!p || !out) return set_error("NULL argument");
+    *out = nullptr;
+    if (!pow2_in_range(p->nx) || !pow2_in_range(p->ny) || !pow2_in_range(p->nz))
+        return set_error("grid %dx%dx%d: each size must be a power of two in "
+                         "4..512", p->nx, p->ny, p->nz);
+    if (!(p->xb > p->xa) || !(p->yb > p->ya) || !(p->zb > p->za))
+        return set_error("invalid spatial bounds, need a < b");
+    if (p->bins < 1 || p->bins > 2048)
+        return set_error("POWER_SPECTRUM_BINS must be in 1..2048");
+    ihg_ctx *c = new ihg_ctx();
+    c->p = *p;
+    c->Nx = p->nx;
+    c->Ny = p->ny;
+    c->Nz = p->nz;
+    c->Mz = p->nz / 2 + 1;
+    c->N = (long)c->Nx * c->Ny * c->Nz;
+    c->M = (long)c->Nx * c->Ny * c->Mz;
+    c->Next = 2 * c->M;
+    c->Ntot = 4 * c->N + 4 * c->Next + 1;
+    // init_grid_pars, src/setup.c:192-208
+    const double TWOPI = 6.283185307179586476925286766559;
+    KGeom &kg = c->kg;
+    kg.Nx = c->Nx;
+    kg.Ny = c->Ny;
+    kg.Nz = c->Nz;
+    kg.Mz = c->Mz;
+    kg.kx1 = TWOPI / (p->xb - p->xa);
+    kg.kx2 = TWOPI * TWOPI / ((p->xb - p->xa) * (p->xb - p->xa));
+    kg.ky1 = TWOPI / (p->yb - p->ya);
+    kg.ky2 = TWOPI * TWOPI / ((p->yb - p->ya) * (p->yb - p->ya));
+    kg.kz1 = TWOPI / (p->zb - p->za);
+    kg.kz2 = TWOPI * TWOPI / ((p->zb - p->za) * (p->zb - p->za));
+    // k_max, src/setup.c:660-664
+    const double k2_max = kg.kx2 * (c->Nx / 2) * (c->Nx / 2) +
+                          kg.ky2 * (c->Ny / 2) * (c->Ny / 2) +
+                          kg.kz2 * (c->Nz / 2) * (c->Nz / 2);
+    kg.k_max = sqrt(k2_max);
+
+    int rc = 0;
+    auto fail = [&](int r) {
+        ihg_destroy(c);
+        return r;
+    };
+    rc = hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking),
+                   "hipStreamCreate");
+    if (rc) {
+        delete c;
+        return rc;
+    }
+    // twiddles: tw[L + m] = e^{-2 pi i m / L} for L = 4..512
+    {
+        double2 *h = (double2 *)calloc(1024, sizeof(double2));
+        for (int L = 4; L <= 512; L *= 2)
+            for (int m = 0; m < L; ++m) {
+                long double ang = -2.0L * 3.14159265358979323846264338327950288L *
+                                  (long double)m / (long double)L;
+                h[L + m] = make_double2((double)cosl(ang), (double)sinl(ang));
+            }
+        if ((rc = dalloc(&c->tw, 1024)) == 0)
+            rc = hip_check(hipMemcpy(c->tw, h, 1024 * sizeof(double2),
+                                     hipMemcpyHostToDevice), "hipMemcpy");
+        free(h);
+        if (rc) return fail(rc);
+    }
+    if (p->enable_filter) {
+        double *h = (double *)malloc(c->M * sizeof(double));
+        host_filter_mask(c, h);
+        if ((rc = dalloc(&c->filter, c->M)) == 0)
+            rc = hip_check(hipMemcpy(c->filter, h, c->M * sizeof(double),
+                                     hipMemcpyHostToDevice), "hipMemcpy");
+        free(h);
+        if (rc) return fail(rc);
+    }
+    if ((rc = dalloc(&c->W, 6 * c->M))) return fail(rc);
+    if ((rc = dalloc(&c->Wc, 2 * c->M))) return fail(rc);
+    if (p->enable_gw && (rc = dalloc(&c->grad3, 3 * c->N))) return fail(rc);
+    // rho | pressure | tmp.f in one allocation (rho and tmp.f 2N apart)
+    if ((rc = dalloc(&c->rho, 3 * c->N))) return fail(rc);
+    c->pressure = c->rho + c->N;
+    c->tmpf = c->rho + 2 * c->N;
+    c->npartials = 3L * 512 * (p->bins > 8 ? p->bins : 8) + 2L * 65536 + 4096;
+    if ((rc = dalloc(&c->partials, c->npartials))) return fail(rc);
+    if ((rc = dalloc(&c->dscal, 16))) return fail(rc);
+    c->obs_len = 28 + 4L * p->bins;
+    if ((rc = dalloc(&c->dobs, c->obs_len))) return fail(rc);
+    if ((rc = hip_check(hipHostMalloc((void **)&c->hscal, 16 * sizeof(double)),
+                        "hipHostMalloc")))
+        return fail(rc);
+    if ((rc = hip_check(hipHostMalloc((void **)&c->hobs,
+                                      c->obs_len * sizeof(double)),
+                        "hipHostMalloc")))
+        return fail(rc);
+    *out = c;
+    return 0;
+}
+
+extern "C" int ihg_sizes(const ihg_ctx *c, size_t out[4])
+{
+    if (!c || !out) return set_error("NULL argument");
+    out[0] = c->N;
+    out[1] = c->M;
+    out[2] = c->Next;
+    out[3] = c->Ntot;
+    return 0;
+}
+
+extern "C" int ihg_malloc(ihg_ctx *c, double **ptr, size_t count)
+{
+    if (!c || !ptr) return set_error("NULL argument");
+    IHG_TRY(hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize"));
+    return dalloc(ptr, count);
+}
+
+extern "C" int ihg_free(ihg_ctx *c, double *ptr)
+{
+    if (!c) return set_error("NULL context");
+    if (!ptr) return 0;
+    IHG_TRY(hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize"));
+    return hip_check(hipFree(ptr), "hipFree");
+}
+
+extern "C" int ihg_upload(ihg_ctx *c, double *dst, const double *src,
+                          size_t count)
+{
+    if (!c || !dst || !src) return set_error("NULL argument");
+    IHG_TRY(hip_check(hipMemcpyAsync(dst, src, count * sizeof(double),
+                                     hipMemcpyHostToDevice, c->stream),
+                      "hipMemcpyAsync H2D"));
+    return hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+}
+
+extern "C" int ihg_download(ihg_ctx *c, double *dst, const double *src,
+                            size_t count)
+{
+    if (!c || !dst || !src) return set_error("NULL argument");
+    IHG_TRY(hip_check(hipMemcpyAsync(dst, src, count * sizeof(double),
+                                     hipMemcpyDeviceToHost, c->stream),
+                      "hipMemcpyAsync D2H"));
+    return hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+}
+
+extern "C" int ihg_copy(ihg_ctx *c, double *dst, const double *src,
+                        size_t count)
+{
+    if (!c || !dst || !src) return set_error("NULL argument");
+    return hip_check(hipMemcpyAsync(dst, src, count * sizeof(double),
+                                    hipMemcpyDeviceToDevice, c->stream),
+                     "hipMemcpyAsync D2D");
+}
+
+extern "C" int ihg_zero(ihg_ctx *c, double *dst, size_t count)
+{
+    if (!c || !dst) return set_error("NULL argument");
+    return hip_check(hipMemsetAsync(dst, 0, count * sizeof(double), c->stream),
+                     "hipMemsetAsync");
+}
+
+extern "C" int ihg_sync(ihg_ctx *c)
+{
+    if (!c) return set_error("NULL context");
+    return hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+}
+
+extern "C" int ihg_fft_r2c(ihg_ctx *c, const double *in, double *out)
+{
+    if (!c) return set_error("NULL context");
+    IHG_TRY(check_ptr(in, "in"));
+    IHG_TRY(check_ptr(out, "out"));
+    return fft_r2c(c, in, reinterpret_cast<double2 *>(out), 1, 0, 0);
+}
+
+extern "C" int ihg_fft_c2r(ihg_ctx *c, const double *in, double *out)
+{
+    if (!c) return set_error("NULL context");
+    IHG_TRY(check_ptr(in, "in"));
+    IHG_TRY(check_ptr(out, "out"));
+    return fft_c2r(c, reinterpret_cast<const double2 *>(in), out, c->Wc, 1, 0,
+                   0);
+}
+
+extern "C" int ihg_set_field(ihg_ctx *c, const double *field)
+{
+    if (!c) return set_error("NULL context");
+    c->field = const_cast<double *>(field);
+    return 0;
+}
+
+extern "C" int ihg_mk_rhs(ihg_ctx *c, double t, double *f, double *result,
+                          unsigned flags)
+{
+    (void)t; // the equations are autonomous (src/toolbox.c:75-88)
+    if (!c) return set_error("NULL context");
+    IHG_TRY(check_ptr(f, "f"));
+    IHG_TRY(check_ptr(result, "result"));
+    if (f == result) return set_error("f and result must differ");
+    c->calls_rhs += 1;
+    const int output = (flags & IHG_OUTPUT) ? 1 : 0;
+    if ((flags & IHG_FILTER) && c->p.enable_filter) IHG_TRY(rhs_filter(c, f));
+    IHG_TRY(rhs_core(c, f, result, output));
+    if (c->p.enable_gw) IHG_TRY(rhs_gw(c, f, result));
+    if (output) IHG_TRY(rhs_output(c, f));
+    return 0;
+}
+
+extern "C" int ihg_means(ihg_ctx *c, double out[2])
+{
+    if (!c || !out) return set_error("NULL argument");
+    IHG_TRY(hip_check(hipMemcpyAsync(c->hscal, c->dscal, 2 * sizeof(double),
+                                     hipMemcpyDeviceToHost, c->stream),
+                      "hipMemcpyAsync"));
+    IHG_TRY(hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize"));
+    out[0] = c->hscal[0];
+    out[1] = c->hscal[1];
+    return 0;
+}
+
+extern "C" int ihg_observables(ihg_ctx *c, double *out, size_t len)
+{
+    if (!c || !out) return set_error("NULL argument");
+    if (!c->have_obs) return set_error("no output call yet");
+    if (len < (size_t)c->obs_len)
+        return set_error("observable buffer needs %ld doubles", c->obs_len);
+    IHG_TRY(hip_check(hipMemcpyAsync(c->hobs, c->dobs,
+                                     c->obs_len * sizeof(double),
+                                     hipMemcpyDeviceToHost, c->stream),
+                      "hipMemcpyAsync"));
+    IHG_TRY(hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize"));
+    memcpy(out, c->hobs, c->obs_len * sizeof(double));
+    return 0;
+}
+
+extern "C" unsigned long ihg_calls_rhs(const ihg_ctx *c)
+{
+    return c ? c->calls_rhs : 0;
+}
+
+extern "C" int ihg_mk_initial_psi(ihg_ctx *c, double ti, double *field,
+                                  double *dfield)
+{
+    if (!c) return set_error("NULL context");
+    IHG_TRY(check_ptr(field, "field"));
+    IHG_TRY(check_ptr(dfield, "dfield"));
+    IHG_TRY(hip_check(hipMemsetAsync(field + 2 * c->N, 0,
+                                     2 * c->N * sizeof(double), c->stream),
+                      "hipMemsetAsync"));
+    IHG_TRY(ihg_mk_rhs(c, ti, field, dfield, 0));
+    return initial_psi(c, field, dfield);
+}
+
+extern "C" int ihg_aux(ihg_ctx *c, int which, double *out)
+{
+    if (!c || !out) return set_error("NULL argument");
+    const double *src = nullptr;
+    if (which >= 0 && which <= 2) {
+        if (!c->grad3) return set_error("gradients are kept only with GW on");
+        src = c->grad3 + which * c->N;
+    } else if (which == 3) {
+        src = c->rho;
+    } else if (which == 4) {
+        src = c->pressure;
+    } else if (which == 5) {
+        src = c->tmpf;
+    } else {
+        return set_error("aux index %d out of range", which);
+    }
+    return ihg_download(c, out, src, c->N);
+}

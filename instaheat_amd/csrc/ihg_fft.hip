@@ -1,0 +1,118 @@
+// ihg_fft.hip -- launchers of the 3D r2c / c2r passes (fft / ifft of
+// src/toolbox.c:806-827).
+#include <type_traits>
+
+#include "dispatch.h"
+#include "fft_kernels.h"
+#include "ihg_internal.h"
+
+namespace ihg {
+
+static LineSet y_lines(const ihg_ctx *c)
+{
+    // lines along y: one per (x, kz); points Mz apart
+    return LineSet{(long)c->Nx * c->Mz, c->Mz, (long)c->Ny * c->Mz, c->Mz};
+}
+
+static LineSet x_lines(const ihg_ctx *c)
+{
+    // lines along x: one per (y, kz); points Ny*Mz apart
+    const long plane = (long)c->Ny * c->Mz;
+    return LineSet{plane, plane, 0, plane};
+}
+
+template <int L>
+static int launch_c2c(ihg_ctx *c, const double2 *in, double2 *out,
+                      LineSet ls, int dir, int batch, long cbatch, int fam)
+{
+    using TL = Tile<L>;
+    dim3 grid((unsigned)((ls.nlines + TL::B - 1) / TL::B), batch);
+    // algorithmic bytes: every complex value read once and written once
+    const double bytes = 2.0 * 16.0 * c->M * batch;
+    return timed(c, fam, bytes, [&]() {
+        if (dir < 0)
+            hipLaunchKernelGGL((k_c2c_strided<L, -1>), grid, dim3(FFT_WG), 0,
+                               c->stream, in, out, ls, cbatch, c->tw);
+        else
+            hipLaunchKernelGGL((k_c2c_strided<L, +1>), grid, dim3(FFT_WG), 0,
+                               c->stream, in, out, ls, cbatch, c->tw);
+        return hip_check(hipGetLastError(), "k_c2c_strided");
+    });
+}
+
+int y_pass(ihg_ctx *c, double2 *data, int dir, int batch, long cbatch)
+{
+    return dispatch_len(c->Ny, [&](auto Lc) {
+        return launch_c2c<decltype(Lc)::value>(c, data, data, y_lines(c), dir,
+                                               batch, cbatch, F_Y_PASS);
+    });
+}
+
+int x_pass(ihg_ctx *c, double2 *data, int dir, int batch, long cbatch)
+{
+    return dispatch_len(c->Nx, [&](auto Lc) {
+        return launch_c2c<decltype(Lc)::value>(c, data, data, x_lines(c), dir,
+                                               batch, cbatch, F_X_PASS);
+    });
+}
+
+static int x_pass_oop(ihg_ctx *c, const double2 *in, double2 *out, int dir,
+                      int batch, long cbatch)
+{
+    return dispatch_len(c->Nx, [&](auto Lc) {
+        return launch_c2c<decltype(Lc)::value>(c, in, out, x_lines(c), dir,
+                                               batch, cbatch, F_X_PASS);
+    });
+}
+
+int z_r2c_pass(ihg_ctx *c, const double *in, double2 *out, int batch,
+               long rbatch, long cbatch)
+{
+    const long nrows = (long)c->Nx * c->Ny;
+    return dispatch_len(c->Nz, [&](auto Lc) {
+        constexpr int L = decltype(Lc)::value;
+        using TL = Tile<L>;
+        dim3 grid((unsigned)((nrows / 2 + TL::B - 1) / TL::B), batch);
+        const double bytes = (8.0 * c->N + 16.0 * c->M) * batch;
+        return timed(c, F_Z_R2C, bytes, [&]() {
+            hipLaunchKernelGGL((k_z_r2c<L>), grid, dim3(FFT_WG), 0, c->stream,
+                               in, out, nrows, rbatch, cbatch, c->tw);
+            return hip_check(hipGetLastError(), "k_z_r2c");
+        });
+    });
+}
+
+int z_c2r_pass(ihg_ctx *c, const double2 *in, double *out, int batch,
+               long cbatch, long rbatch)
+{
+    const long nrows = (long)c->Nx * c->Ny;
+    return dispatch_len(c->Nz, [&](auto Lc) {
+        constexpr int L = decltype(Lc)::value;
+        using TL = Tile<L>;
+        dim3 grid((unsigned)((nrows / 2 + TL::B - 1) / TL::B), batch);
+        const double bytes = (8.0 * c->N + 16.0 * c->M) * batch;
+        return timed(c, F_Z_C2R, bytes, [&]() {
+            hipLaunchKernelGGL((k_z_c2r<L>), grid, dim3(FFT_WG), 0, c->stream,
+                               in, out, nrows, cbatch, rbatch, c->tw);
+            return hip_check(hipGetLastError(), "k_z_c2r");
+        });
+    });
+}
+
+int fft_r2c(ihg_ctx *c, const double *in, double2 *out, int batch,
+            long rbatch, long cbatch)
+{
+    IHG_TRY(z_r2c_pass(c, in, out, batch, rbatch, cbatch));
+    IHG_TRY(y_pass(c, out, -1, batch, cbatch));
+    return x_pass(c, out, -1, batch, cbatch);
+}
+
+int fft_c2r(ihg_ctx *c, const double2 *in, double *out, double2 *work,
+            int batch, long cbatch, long rbatch)
+{
+    IHG_TRY(x_pass_oop(c, in, work, +1, batch, cbatch));
+    IHG_TRY(y_pass(c, work, +1, batch, cbatch));
+    return z_c2r_pass(c, work, out, batch, cbatch, rbatch);
+}
+
+} // namespace ihg

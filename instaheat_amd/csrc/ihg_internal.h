@@ -1,0 +1,157 @@
+// ihg_internal.h -- library-private declarations shared by the .hip units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <vector>
+
+#include "../../include/ihg.h"
+
+namespace ihg {
+
+// k-space geometry, passed by value to kernels.  The wave numbers are
+// derived from the grid indices exactly as mk_k_grid does
+// (src/setup.c:609-666), so no k tables are read from HBM.
+struct KGeom {
+    int Nx, Ny, Nz, Mz;
+    double kx1, ky1, kz1; // 2 pi / L_i
+    double kx2, ky2, kz2; // (2 pi / L_i)^2
+    double k_max;
+
+    // derivative wave numbers: Nyquist entry zeroed (kvec.x/y/z)
+    __device__ __forceinline__ double kxd(int i) const
+    {
+        return (i > Nx / 2) ? kx1 * (i - Nx) : ((2 * i == Nx) ? 0.0 : kx1 * i);
+    }
+    __device__ __forceinline__ double kyd(int j) const
+    {
+        return (j > Ny / 2) ? ky1 * (j - Ny) : ((2 * j == Ny) ? 0.0 : ky1 * j);
+    }
+    __device__ __forceinline__ double kzd(int k) const
+    {
+        return (2 * k == Nz) ? 0.0 : kz1 * k;
+    }
+    // full wave numbers (kvec.xf/yf/zf)
+    __device__ __forceinline__ double kxf(int i) const
+    {
+        return (i > Nx / 2) ? kx1 * (i - Nx) : kx1 * i;
+    }
+    __device__ __forceinline__ double kyf(int j) const
+    {
+        return (j > Ny / 2) ? ky1 * (j - Ny) : ky1 * j;
+    }
+    __device__ __forceinline__ double kzf(int k) const { return kz1 * k; }
+    // |k|^2 (kvec.sq), same operation order as the reference, no contraction
+    __device__ __forceinline__ double ksq(int i, int j, int k) const
+    {
+#pragma clang fp contract(off)
+        double k2 = kz2 * (double)k * (double)k;
+        const double ii = (i > Nx / 2) ? (double)(Nx - i) : (double)i;
+        const double jj = (j > Ny / 2) ? (double)(Ny - j) : (double)j;
+        k2 += kx2 * ii * ii;
+        k2 += ky2 * jj * jj;
+        return k2;
+    }
+};
+
+} // namespace ihg
+
+namespace ihg {
+// kernel families timed by the live profiler (ihg_prof_*)
+enum Fam {
+    F_Z_R2C, F_Z_C2R, F_Y_PASS, F_X_PASS, F_X_FANOUT, F_Z_C2R_RHO, F_MEANS,
+    F_UPDATE, F_Z_R2C_SRC, F_TT_UPDATE_H, F_X_ROUNDTRIP, F_OBS, F_DP_STAGE,
+    F_DP_FINISH, F_RK4, F_NFAM
+};
+struct ProfRec {
+    int fam;
+    hipEvent_t a, b;
+    double bytes;
+};
+} // namespace ihg
+
+struct ihg_ctx {
+    ihg_params p;
+    int Nx, Ny, Nz, Mz;
+    long N, M, Next, Ntot;
+    ihg::KGeom kg;
+    hipStream_t stream;
+    double2 *tw;       // twiddle tables, tw[L + m] = e^{-2 pi i m / L}
+    double *filter;    // filter mask, M values (filter in the reference)
+    double2 *W;        // 6 spectral work arrays (M complex each)
+    double2 *Wc;       // 2 more spectral arrays (phic / psic for output)
+    double *grad3;     // d_x phi, d_y phi, d_z phi (N each) for GW sources
+    double *rho;       // N
+    double *pressure;  // N
+    double *tmpf;      // N, Laplacian of psi on output steps (tmp.f)
+    double *partials;  // per-block reduction partials
+    long npartials;
+    double *dscal;     // device scalars: [0] rho_mean, [1] pressure_mean, ...
+    double *hscal;     // pinned host mirror of dscal
+    double *dobs;      // device observables (packed)
+    double *hobs;      // pinned host copy of the observables
+    long obs_len;
+    double *field;     // array mk_summary reads (the reference's `field`)
+    unsigned long calls_rhs;
+    int have_obs;
+    // live profiler: events around every launch of the timed families
+    bool prof_on = false;
+    std::vector<ihg::ProfRec> prof_pending;
+    std::vector<hipEvent_t> ev_pool;
+    double fam_ms[ihg::F_NFAM] = {};
+    double fam_bytes[ihg::F_NFAM] = {};
+    long fam_count[ihg::F_NFAM] = {};
+};
+
+namespace ihg {
+
+// FFT launchers (ihg_fft.hip)
+int fft_r2c(ihg_ctx *c, const double *in, double2 *out, int batch,
+            long rbatch, long cbatch);
+int fft_c2r(ihg_ctx *c, const double2 *in, double *out, double2 *work,
+            int batch, long cbatch, long rbatch);
+int y_pass(ihg_ctx *c, double2 *data, int dir, int batch, long cbatch);
+int x_pass(ihg_ctx *c, double2 *data, int dir, int batch, long cbatch);
+int z_r2c_pass(ihg_ctx *c, const double *in, double2 *out, int batch,
+               long rbatch, long cbatch);
+int z_c2r_pass(ihg_ctx *c, const double2 *in, double *out, int batch,
+               long cbatch, long rbatch);
+
+// right-hand side launchers (ihg_rhs.hip)
+int rhs_core(ihg_ctx *c, double *f, double *result, int output);
+int rhs_filter(ihg_ctx *c, double *f);
+int rhs_gw(ihg_ctx *c, const double *f, double *result);
+int rhs_output(ihg_ctx *c, const double *f);
+int initial_psi(ihg_ctx *c, double *field, double *dfield);
+
+int set_error(const char *fmt, ...);
+int hip_check(hipError_t e, const char *what);
+
+// profiler hooks (ihg_api.hip): bracket one launch on the context stream
+int prof_begin(ihg_ctx *c, int fam, double bytes);
+int prof_end(ihg_ctx *c);
+
+// launch(F) between two events when the profiler is on
+template <class F>
+int timed(ihg_ctx *c, int fam, double bytes, F &&launch)
+{
+    if (c->prof_on) {
+        int rc = prof_begin(c, fam, bytes);
+        if (rc) return rc;
+    }
+    int rc = launch();
+    if (rc) return rc;
+    return c->prof_on ? prof_end(c) : 0;
+}
+
+} // namespace ihg
+
+#define IHG_TRY(expr)                                                         \
+    do {                                                                      \
+        int _rc = (expr);                                                     \
+        if (_rc) return _rc;                                                  \
+    } while (0)
+
+#define IHG_HIP(expr) IHG_TRY(::ihg::hip_check((expr), #expr))
+#define IHG_LAUNCHED(name) IHG_HIP(hipGetLastError())

@@ -1,0 +1,320 @@
+// ihg_step.hip -- stage arithmetic of run_rk4 (src/rk4.c:78-102) and of
+// try_step / error (src/dopri853.c:331-486).
+//
+// All stage kernels stream the live part of the state: [0, 4N) plus, with
+// GW on, the four h blocks; the scale factor a (last element) is handled by
+// thread 0.  With GW off the h blocks of every stepper array stay zero (the
+// reference zeroes its DOPRI853 buffers, src/dopri853.c:610-624, and never
+// writes them), so skipping them changes no value; the error norm still
+// divides by Ntot.
+#include <cmath>
+
+#include "dp853_tableau.h"
+#include "ihg_internal.h"
+#include "reduce.h"
+
+namespace ihg {
+
+struct StageIn {
+    const double *k[12];
+    double a[12];
+    int nk;
+};
+
+static inline long live_len(const ihg_ctx *c)
+{
+    return c->p.enable_gw ? c->Ntot - 1 : 4 * c->N;
+}
+
+static inline unsigned stream_blocks(long npairs)
+{
+    long b = (npairs + RED_WG - 1) / RED_WG;
+    if (b < 1) b = 1;
+    return (unsigned)(b > 262144 ? 262144 : b);
+}
+
+// out = y + dt * sum_j a_j k_j   (try_step's stage loops)
+__global__ __launch_bounds__(RED_WG) void k_dp_stage(
+    double *__restrict__ out, const double *__restrict__ y, StageIn s,
+    double dt, long nlive, long Ntot)
+{
+    const long npairs = nlive / 2;
+    const long stride = (long)gridDim.x * RED_WG;
+    const double2 *y2 = reinterpret_cast<const double2 *>(y);
+    double2 *o2 = reinterpret_cast<double2 *>(out);
+    for (long i = (long)blockIdx.x * RED_WG + threadIdx.x; i < npairs;
+         i += stride) {
+        const double2 k0 = reinterpret_cast<const double2 *>(s.k[0])[i];
+        double ax = s.a[0] * k0.x, ay = s.a[0] * k0.y;
+#pragma unroll
+        for (int j = 1; j < 12; ++j) {
+            if (j < s.nk) {
+                const double2 kj = reinterpret_cast<const double2 *>(s.k[j])[i];
+                ax += s.a[j] * kj.x;
+                ay += s.a[j] * kj.y;
+            }
+        }
+        const double2 yy = y2[i];
+        o2[i] = make_double2(yy.x + dt * ax, yy.y + dt * ay);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const long i = Ntot - 1;
+        double acc = s.a[0] * s.k[0][i];
+        for (int j = 1; j < s.nk; ++j)
+            acc += s.a[j] * s.k[j][i];
+        out[i] = y[i] + dt * acc;
+    }
+}
+
+// RK4 stage: out = y + dt * k / 2 (half) or y + dt * k
+__global__ __launch_bounds__(RED_WG) void k_rk4_stage(
+    double *__restrict__ out, const double *__restrict__ y,
+    const double *__restrict__ k, double dt, int half, long nlive, long Ntot)
+{
+    const long npairs = nlive / 2;
+    const long stride = (long)gridDim.x * RED_WG;
+    const double2 *y2 = reinterpret_cast<const double2 *>(y);
+    const double2 *k2 = reinterpret_cast<const double2 *>(k);
+    double2 *o2 = reinterpret_cast<double2 *>(out);
+    for (long i = (long)blockIdx.x * RED_WG + threadIdx.x; i < npairs;
+         i += stride) {
+        const double2 yy = y2[i], kk = k2[i];
+        if (half)
+            o2[i] = make_double2(yy.x + dt * kk.x / 2.0, yy.y + dt * kk.y / 2.0);
+        else
+            o2[i] = make_double2(yy.x + dt * kk.x, yy.y + dt * kk.y);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const long i = Ntot - 1;
+        out[i] = half ? y[i] + dt * k[i] / 2.0 : y[i] + dt * k[i];
+    }
+}
+
+// RK4 update: y += dt * (k1 + 2 k2 + 2 k3 + k4) / 6
+__global__ __launch_bounds__(RED_WG) void k_rk4_final(
+    double *__restrict__ y, const double *__restrict__ k1,
+    const double *__restrict__ k2, const double *__restrict__ k3,
+    const double *__restrict__ k4, double dt, long nlive, long Ntot)
+{
+    const long npairs = nlive / 2;
+    const long stride = (long)gridDim.x * RED_WG;
+    double2 *y2 = reinterpret_cast<double2 *>(y);
+    for (long i = (long)blockIdx.x * RED_WG + threadIdx.x; i < npairs;
+         i += stride) {
+        const double2 a = reinterpret_cast<const double2 *>(k1)[i];
+        const double2 b = reinterpret_cast<const double2 *>(k2)[i];
+        const double2 c = reinterpret_cast<const double2 *>(k3)[i];
+        const double2 d = reinterpret_cast<const double2 *>(k4)[i];
+        double2 yy = y2[i];
+        yy.x += dt * (a.x + 2.0 * b.x + 2.0 * c.x + d.x) / 6.0;
+        yy.y += dt * (a.y + 2.0 * b.y + 2.0 * c.y + d.y) / 6.0;
+        y2[i] = yy;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const long i = Ntot - 1;
+        y[i] += dt * (k1[i] + 2.0 * k2[i] + 2.0 * k3[i] + k4[i]) / 6.0;
+    }
+}
+
+struct FinishIn {
+    const double *k[12];
+};
+
+__device__ __forceinline__ void dp_finish_point(const FinishIn &f, long i,
+                                                double yv, double dt,
+                                                double rtol, double atol,
+                                                double &ynew, double &e5,
+                                                double &e3)
+{
+    const double k0 = f.k[0][i], k5 = f.k[5][i], k6 = f.k[6][i],
+                 k7 = f.k[7][i], k8 = f.k[8][i], k9 = f.k[9][i],
+                 k10 = f.k[10][i], k11 = f.k[11][i];
+    const double bs = DP_B[0] * k0 + DP_B[5] * k5 + DP_B[6] * k6 +
+                      DP_B[7] * k7 + DP_B[8] * k8 + DP_B[9] * k9 +
+                      DP_B[10] * k10 + DP_B[11] * k11;
+    ynew = yv + dt * bs;
+    const double yerr = bs - DP_BHH[0] * k0 - DP_BHH[1] * k8 - DP_BHH[2] * k11;
+    const double yerr2 = DP_ER[0] * k0 + DP_ER[5] * k5 + DP_ER[6] * k6 +
+                         DP_ER[7] * k7 + DP_ER[8] * k8 + DP_ER[9] * k9 +
+                         DP_ER[10] * k10 + DP_ER[11] * k11;
+    const double sk = atol + rtol * fmax(fabs(yv), fabs(ynew));
+    double t = yerr / sk;
+    e3 += t * t;
+    t = yerr2 / sk;
+    e5 += t * t;
+}
+
+// b-weighted update, embedded errors and the two sums of error()
+__global__ __launch_bounds__(RED_WG) void k_dp_finish(
+    const double *__restrict__ y, FinishIn f, double *__restrict__ ynew,
+    double dt, double rtol, double atol, long nlive, long Ntot,
+    double *__restrict__ part)
+{
+    __shared__ double sh[8];
+    double e5 = 0.0, e3 = 0.0;
+    const long stride = (long)gridDim.x * RED_WG;
+    for (long i = (long)blockIdx.x * RED_WG + threadIdx.x; i < nlive;
+         i += stride) {
+        double yn;
+        dp_finish_point(f, i, y[i], dt, rtol, atol, yn, e5, e3);
+        ynew[i] = yn;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        double yn;
+        dp_finish_point(f, Ntot - 1, y[Ntot - 1], dt, rtol, atol, yn, e5, e3);
+        ynew[Ntot - 1] = yn;
+    }
+    e5 = block_reduce<OpSum>(e5, sh, RED_WG);
+    e3 = block_reduce<OpSum>(e3, sh, RED_WG);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = e5;
+        part[2 * blockIdx.x + 1] = e3;
+    }
+}
+
+__global__ __launch_bounds__(RED_WG) void k_sum2_final(
+    const double *__restrict__ part, int nb, double *__restrict__ out)
+{
+    __shared__ double sh[8];
+    double s0 = 0.0, s1 = 0.0;
+    for (int b = threadIdx.x; b < nb; b += RED_WG) {
+        s0 += part[2 * b];
+        s1 += part[2 * b + 1];
+    }
+    s0 = block_reduce<OpSum>(s0, sh, RED_WG);
+    s1 = block_reduce<OpSum>(s1, sh, RED_WG);
+    if (threadIdx.x == 0) {
+        out[0] = s0;
+        out[1] = s1;
+    }
+}
+
+} // namespace ihg
+
+using namespace ihg;
+
+static int check_ptr(const void *p, const char *name)
+{
+    if (!p) return set_error("%s is NULL", name);
+    if (((size_t)p) & 15)
+        return set_error("%s is not 16-byte aligned", name);
+    return 0;
+}
+
+extern "C" int ihg_rk4_stage(ihg_ctx *c, double *out, const double *y,
+                             const double *k, double dt, int half)
+{
+    if (!c) return set_error("NULL context");
+    IHG_TRY(check_ptr(out, "out"));
+    IHG_TRY(check_ptr(y, "y"));
+    IHG_TRY(check_ptr(k, "k"));
+    const long nl = live_len(c);
+    return timed(c, F_RK4, 3.0 * 8.0 * nl, [&]() {
+        hipLaunchKernelGGL(k_rk4_stage, dim3(stream_blocks(nl / 2)),
+                           dim3(RED_WG), 0, c->stream, out, y, k, dt,
+                           half ? 1 : 0, nl, c->Ntot);
+        return hip_check(hipGetLastError(), "k_rk4_stage");
+    });
+}
+
+extern "C" int ihg_rk4_final(ihg_ctx *c, double *y, const double *k1,
+                             const double *k2, const double *k3,
+                             const double *k4, double dt)
+{
+    if (!c) return set_error("NULL context");
+    IHG_TRY(check_ptr(y, "y"));
+    IHG_TRY(check_ptr(k1, "k1"));
+    IHG_TRY(check_ptr(k2, "k2"));
+    IHG_TRY(check_ptr(k3, "k3"));
+    IHG_TRY(check_ptr(k4, "k4"));
+    const long nl = live_len(c);
+    return timed(c, F_RK4, 6.0 * 8.0 * nl, [&]() {
+        hipLaunchKernelGGL(k_rk4_final, dim3(stream_blocks(nl / 2)),
+                           dim3(RED_WG), 0, c->stream, y, k1, k2, k3, k4, dt,
+                           nl, c->Ntot);
+        return hip_check(hipGetLastError(), "k_rk4_final");
+    });
+}
+
+extern "C" int ihg_dp853_stage(ihg_ctx *c, int s, double dt, const double *y,
+                               const double *const *k, double *out)
+{
+    if (!c || !k) return set_error("NULL argument");
+    if (s < 2 || s > 12) return set_error("stage %d out of range 2..12", s);
+    IHG_TRY(check_ptr(y, "y"));
+    IHG_TRY(check_ptr(out, "out"));
+    StageIn in{};
+    int n = 0;
+    for (int j = 0; j < s - 1; ++j) {
+        if (DP_A[s - 1][j] == 0.0) continue;
+        IHG_TRY(check_ptr(k[j], "k[j]"));
+        in.k[n] = k[j];
+        in.a[n] = DP_A[s - 1][j];
+        ++n;
+    }
+    in.nk = n;
+    const long nl = live_len(c);
+    return timed(c, F_DP_STAGE, (n + 2.0) * 8.0 * nl, [&]() {
+        hipLaunchKernelGGL(k_dp_stage, dim3(stream_blocks(nl / 2)),
+                           dim3(RED_WG), 0, c->stream, out, y, in, dt, nl,
+                           c->Ntot);
+        return hip_check(hipGetLastError(), "k_dp_stage");
+    });
+}
+
+extern "C" int ihg_dp853_finish(ihg_ctx *c, double dt, const double *y,
+                                const double *const *k, double *y_new,
+                                double rtol, double atol, double sums[2])
+{
+    if (!c || !k || !sums) return set_error("NULL argument");
+    IHG_TRY(check_ptr(y, "y"));
+    IHG_TRY(check_ptr(y_new, "y_new"));
+    FinishIn in{};
+    for (int j = 0; j < 12; ++j) {
+        if (j >= 1 && j <= 4) continue; // stages 2..5 carry no b / er weight
+        IHG_TRY(check_ptr(k[j], "k[j]"));
+        in.k[j] = k[j];
+    }
+    for (int j = 1; j <= 4; ++j)
+        in.k[j] = k[0];
+    const long nl = live_len(c);
+    long nb = (nl + RED_WG * 8 - 1) / (RED_WG * 8);
+    if (nb > 2048) nb = 2048;
+    if (nb < 1) nb = 1;
+    if (2 * nb + 2 > c->npartials) return set_error("partials buffer too small");
+    // y and the 8 weighted stages in, y_new out
+    IHG_TRY(timed(c, F_DP_FINISH, 10.0 * 8.0 * nl, [&]() {
+        hipLaunchKernelGGL(k_dp_finish, dim3((unsigned)nb), dim3(RED_WG), 0,
+                           c->stream, y, in, y_new, dt, rtol, atol, nl,
+                           c->Ntot, c->partials);
+        return hip_check(hipGetLastError(), "k_dp_finish");
+    }));
+    hipLaunchKernelGGL(k_sum2_final, dim3(1), dim3(RED_WG), 0, c->stream,
+                       c->partials, (int)nb, c->dscal + 8);
+    IHG_LAUNCHED("k_sum2_final");
+    IHG_HIP(hipMemcpyAsync(c->hscal + 8, c->dscal + 8, 2 * sizeof(double),
+                           hipMemcpyDeviceToHost, c->stream));
+    IHG_HIP(hipStreamSynchronize(c->stream));
+    sums[0] = c->hscal[8];
+    sums[1] = c->hscal[9];
+    return 0;
+}
+
+extern "C" int ihg_dp853_tableau(double *a, double *cc, double *b, double *bhh,
+                                 double *er)
+{
+    if (!a || !cc || !b || !bhh || !er) return set_error("NULL argument");
+    for (int s = 0; s < 13; ++s)
+        for (int j = 0; j < 12; ++j)
+            a[s * 12 + j] = (s < 12) ? DP_A[s][j] : DP_B[j];
+    for (int s = 0; s < 12; ++s)
+        cc[s] = DP_C[s];
+    cc[12] = 1.0;
+    for (int j = 0; j < 12; ++j) {
+        b[j] = DP_B[j];
+        er[j] = DP_ER[j];
+    }
+    for (int j = 0; j < 3; ++j)
+        bhh[j] = DP_BHH[j];
+    return 0;
+}

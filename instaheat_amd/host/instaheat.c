@@ -1,0 +1,476 @@
+/*
+ * instaheat.c -- C99 host of the InstaHeat hot path (include/instaheat.h).
+ *
+ * The control flow is the reference's; every array operation is a libihg
+ * call on device memory.  Per accepted DOPRI853 step the host reads back
+ * only the two error sums of each attempt and rho_mean (for the Hubble cap).
+ */
+#define _POSIX_C_SOURCE 199309L
+#include "instaheat.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#define IH_MAX(x, y) (((x) > (y)) ? (x) : (y))
+#define IH_MIN(x, y) (((x) > (y)) ? (y) : (x))
+
+struct ih_sim {
+    ih_config cfg;
+    ihg_ctx *g;
+    size_t N, M, Next, Ntot;
+    double *field, *dfield, *field_new, *dfield_new; /* device */
+    double *stage[12];   /* stage RHS arrays: [1..11] used by DOPRI853,
+                            [0..3] are k1..k4 of RK4 */
+    double *k_tmp;       /* stage input */
+    struct ih_evolution_flags evo_flags;
+    double t;
+    double *obs;         /* host copy of the observables */
+    size_t obs_len;
+    double dp_c[13];     /* DOPRI853 nodes c_s */
+};
+
+static double wall_time(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+void ih_config_defaults(ih_config *cfg)
+{
+    memset(cfg, 0, sizeof *cfg);
+    cfg->grid.nx = cfg->grid.ny = cfg->grid.nz = 32;
+    cfg->grid.xa = cfg->grid.ya = cfg->grid.za = -5.0;
+    cfg->grid.xb = cfg->grid.yb = cfg->grid.zb = 5.0;
+    cfg->grid.mass = 4.0;
+    cfg->grid.enable_gw = 1;
+    cfg->grid.enable_filter = 1;
+    cfg->grid.bins = 50;
+    cfg->integration_method = IH_DOPRI853;
+    cfg->ti = 0.0;
+    cfg->tf = 1.0;
+    cfg->dt = 1.0e-7;
+    cfg->dt_min = 1.0e-9;
+    cfg->max_steps = (long)1e15;
+    cfg->beta = 0.0;
+    cfg->safe = 0.9;
+    cfg->minscale = 0.333;
+    cfg->maxscale = 3.0;
+    cfg->rtol = 1.0e-10;
+    cfg->atol = 1.0e-14;
+    cfg->max_dt_hubble_fraction = 1.0e-2;
+    cfg->skip = 1;
+    cfg->a_initial = 1.0;
+}
+
+void ih_destroy(ih_sim *s)
+{
+    if (!s) return;
+    if (s->g) {
+        double *arrs[] = {s->field, s->dfield, s->field_new, s->dfield_new,
+                          s->k_tmp};
+        for (size_t i = 0; i < sizeof arrs / sizeof arrs[0]; ++i)
+            ihg_free(s->g, arrs[i]);
+        for (int i = 0; i < 12; ++i) ihg_free(s->g, s->stage[i]);
+        ihg_destroy(s->g);
+    }
+    free(s->obs);
+    free(s);
+}
+
+int ih_create(const ih_config *cfg, ih_sim **out)
+{
+    if (!cfg || !out) return IHG_EINVAL;
+    *out = NULL;
+    if (cfg->skip < 1 || cfg->dt <= 0.0 || cfg->ti > cfg->tf) return IHG_EINVAL;
+    if (cfg->integration_method != IH_RK4 &&
+        cfg->integration_method != IH_DOPRI853)
+        return IHG_EINVAL;
+    ih_sim *s = calloc(1, sizeof *s);
+    if (!s) return IHG_ENOMEM;
+    s->cfg = *cfg;
+    int rc = ihg_create(&cfg->grid, &s->g);
+    if (rc) {
+        free(s);
+        return rc;
+    }
+    size_t sz[4];
+    ihg_sizes(s->g, sz);
+    s->N = sz[0];
+    s->M = sz[1];
+    s->Next = sz[2];
+    s->Ntot = sz[3];
+    double **arrs[] = {&s->field, &s->dfield, &s->field_new, &s->dfield_new,
+                       &s->k_tmp};
+    for (size_t i = 0; i < sizeof arrs / sizeof arrs[0] && !rc; ++i)
+        rc = ihg_malloc(s->g, arrs[i], s->Ntot);
+    const int nstage = cfg->integration_method == IH_RK4 ? 4 : 12;
+    for (int i = 0; i < nstage && !rc; ++i)
+        rc = ihg_malloc(s->g, &s->stage[i], s->Ntot);
+    s->obs_len = 28 + 4 * (size_t)cfg->grid.bins;
+    s->obs = calloc(s->obs_len, sizeof(double));
+    if (!rc && !s->obs) rc = IHG_ENOMEM;
+    if (!rc) {
+        double a[13 * 12], b[12], bhh[3], er[12];
+        rc = ihg_dp853_tableau(a, s->dp_c, b, bhh, er);
+    }
+    if (rc) {
+        ih_destroy(s);
+        return rc;
+    }
+    s->t = cfg->ti;
+    ihg_set_field(s->g, s->field);
+    *out = s;
+    return 0;
+}
+
+ihg_ctx *ih_gpu(ih_sim *s) { return s->g; }
+const ih_config *ih_cfg(const ih_sim *s) { return &s->cfg; }
+double *ih_field(ih_sim *s) { return s->field; }
+double *ih_dfield(ih_sim *s) { return s->dfield; }
+struct ih_evolution_flags *ih_flags(ih_sim *s) { return &s->evo_flags; }
+double ih_time(const ih_sim *s) { return s->t; }
+void ih_set_time(ih_sim *s, double t) { s->t = t; }
+void ih_set_dt(ih_sim *s, double dt) { s->cfg.dt = dt; }
+void ih_set_max_steps(ih_sim *s, long max_steps) { s->cfg.max_steps = max_steps; }
+
+int ih_mk_rhs(ih_sim *s, double t, double *f, double *result)
+{
+    unsigned fl = (s->evo_flags.filter ? IHG_FILTER : 0u) |
+                  (s->evo_flags.output ? IHG_OUTPUT : 0u);
+    return ihg_mk_rhs(s->g, t, f, result, fl);
+}
+
+/* save(), src/io.c:494-613: a is read from `field` */
+static int do_save(ih_sim *s, ih_save_fn save, void *user, double t)
+{
+    if (!save) return 0;
+    double a;
+    int rc = ihg_observables(s->g, s->obs, s->obs_len);
+    if (!rc) rc = ihg_download(s->g, &a, s->field + s->Ntot - 1, 1);
+    if (rc) return rc;
+    save(user, t, a, s->obs, s->obs_len);
+    return 0;
+}
+
+/* prepare_and_save_timeslice, src/toolbox.c:96-104 */
+static int prepare_and_save(ih_sim *s, ih_save_fn save, void *user)
+{
+    s->evo_flags.output = 1;
+    s->evo_flags.filter = 1;
+    ihg_set_field(s->g, s->field);
+    int rc = ih_mk_rhs(s, s->t, s->field, s->dfield);
+    s->evo_flags.filter = 0;
+    s->evo_flags.output = 0;
+    if (rc) return rc;
+    return do_save(s, save, user, s->t);
+}
+
+/* ------------------------------------------------------------------------ */
+/* initial data                                                             */
+/* ------------------------------------------------------------------------ */
+
+static uint64_t splitmix64(uint64_t x)
+{
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+/* uniform in (0, 1] from 53 random bits */
+static double unit_open0(uint64_t r)
+{
+    return ((double)(r >> 11) + 1.0) * (1.0 / 9007199254740992.0);
+}
+
+int ih_init_gaussian(ih_sim *s, uint64_t seed, double amplitude, double phi0,
+                     double dphi0, double k_cut)
+{
+    const ihg_params *p = &s->cfg.grid;
+    const int Nx = p->nx, Ny = p->ny, Nz = p->nz, Mz = Nz / 2 + 1;
+    const double TWOPI = 6.283185307179586476925286766559;
+    const double kx1 = TWOPI / (p->xb - p->xa), ky1 = TWOPI / (p->yb - p->ya),
+                 kz1 = TWOPI / (p->zb - p->za);
+    const double a0 = s->cfg.a_initial;
+    const double m2 = p->mass * p->mass;
+    double *spec = calloc(2 * s->M, sizeof(double));
+    double *real = calloc(s->Ntot, sizeof(double));
+    if (!spec || !real) {
+        free(spec);
+        free(real);
+        return IHG_ENOMEM;
+    }
+    for (int i = 0; i < Nx; ++i) {
+        const double kx = kx1 * (i > Nx / 2 ? i - Nx : i);
+        for (int j = 0; j < Ny; ++j) {
+            const double ky = ky1 * (j > Ny / 2 ? j - Ny : j);
+            for (int k = 0; k < Mz; ++k) {
+                const double kz = kz1 * k;
+                const double k2 = kx * kx + ky * ky + kz * kz;
+                if (k2 == 0.0 || k2 > k_cut * k_cut) continue;
+                const size_t id = ((size_t)i * Ny + j) * Mz + k;
+                const uint64_t base = splitmix64(seed ^ 0x5DEECE66Dull) + 2 * id;
+                const double u1 = unit_open0(splitmix64(base));
+                const double u2 = unit_open0(splitmix64(base + 1)) - 0x1p-53;
+                const double omega = sqrt(k2 + m2);
+                const double sigma =
+                    amplitude * sqrt(1.0 / (2.0 * a0 * a0 * a0 * omega));
+                const double r = sqrt(-2.0 * log(u1)) * sigma / sqrt(2.0);
+                spec[2 * id] = r * cos(TWOPI * u2);
+                spec[2 * id + 1] = r * sin(TWOPI * u2);
+            }
+        }
+    }
+    int rc = ihg_zero(s->g, s->field, s->Ntot);
+    if (!rc) rc = ihg_zero(s->g, s->dfield, s->Ntot);
+    if (!rc) rc = ihg_upload(s->g, s->k_tmp, spec, 2 * s->M);
+    if (!rc) rc = ihg_fft_c2r(s->g, s->k_tmp, s->field);
+    if (!rc) rc = ihg_download(s->g, real, s->field, s->N);
+    if (!rc) {
+        for (size_t i = 0; i < s->N; ++i) {
+            real[i] += phi0;
+            real[s->N + i] = dphi0;
+        }
+        real[s->Ntot - 1] = a0;
+        rc = ihg_upload(s->g, s->field, real, s->Ntot);
+    }
+    if (!rc) rc = ihg_zero(s->g, s->k_tmp, s->Ntot);
+    free(spec);
+    free(real);
+    s->t = s->cfg.ti;
+    return rc;
+}
+
+int ih_mk_initial_psi(ih_sim *s)
+{
+    s->evo_flags.filter = 0;
+    s->evo_flags.output = 0;
+    ihg_set_field(s->g, s->field);
+    return ihg_mk_initial_psi(s->g, s->cfg.ti, s->field, s->dfield);
+}
+
+/* ------------------------------------------------------------------------ */
+/* RK4, src/rk4.c:27-132                                                    */
+/* ------------------------------------------------------------------------ */
+
+long ih_run_rk4(ih_sim *s, ih_save_fn save, void *user)
+{
+    const ih_config *c = &s->cfg;
+    double dt = c->dt, t = s->t;
+    double *k1 = s->stage[0], *k2 = s->stage[1], *k3 = s->stage[2],
+           *k4 = s->stage[3], *tmp = s->k_tmp;
+    int rc = 0;
+    long nt;
+    if (!k4) return IHG_EINVAL; /* created for DOPRI853 only */
+    ihg_set_field(s->g, s->field);
+    for (nt = 0; t < c->tf && nt < c->max_steps; ++nt) {
+        if (t + dt * 1.0001 > c->tf) dt = c->tf - t;
+        s->evo_flags.filter = c->grid.enable_filter ? 1 : 0;
+        if (nt % c->skip == 0) {
+            s->evo_flags.output = 1;
+            rc = ih_mk_rhs(s, t, s->field, k1);
+            s->evo_flags.output = 0;
+            s->t = t;
+            if (!rc) rc = do_save(s, save, user, t);
+        } else {
+            rc = ih_mk_rhs(s, t, s->field, k1);
+        }
+        s->evo_flags.filter = 0;
+        if (!rc) rc = ihg_rk4_stage(s->g, tmp, s->field, k1, dt, 1);
+        if (!rc) rc = ih_mk_rhs(s, t + dt / 2.0, tmp, k2);
+        if (!rc) rc = ihg_rk4_stage(s->g, tmp, s->field, k2, dt, 1);
+        if (!rc) rc = ih_mk_rhs(s, t + dt / 2.0, tmp, k3);
+        if (!rc) rc = ihg_rk4_stage(s->g, tmp, s->field, k3, dt, 0);
+        if (!rc) rc = ih_mk_rhs(s, t + dt, tmp, k4);
+        if (!rc) rc = ihg_rk4_final(s->g, s->field, k1, k2, k3, k4, dt);
+        if (rc) return rc;
+        t += dt;
+    }
+    s->t = t;
+    rc = prepare_and_save(s, save, user);
+    if (!rc) rc = ihg_sync(s->g);
+    return rc ? rc : nt;
+}
+
+/* ------------------------------------------------------------------------ */
+/* DOPRI853, src/dopri853.c                                                 */
+/* ------------------------------------------------------------------------ */
+
+int ih_success(ih_ctrl *ctl, const ih_config *cfg, double err, double *dt,
+               double rho_mean)
+{
+    const double alpha = 1.0 / 8.0 - cfg->beta * 0.2;
+    double scale;
+    if (err <= 1.0) {
+        if (err == 0.0) {
+            scale = cfg->maxscale;
+        } else {
+            scale = cfg->safe * pow(err, -alpha) * pow(ctl->err_old, cfg->beta);
+            if (scale < cfg->minscale) scale = cfg->minscale;
+            if (scale > cfg->maxscale) scale = cfg->maxscale;
+        }
+        if (ctl->reject)
+            ctl->dt_next = (*dt) * IH_MIN(scale, 1.0);
+        else
+            ctl->dt_next = (*dt) * scale;
+        if (cfg->max_dt_hubble_fraction > 0.0) {
+            const double minstep =
+                cfg->max_dt_hubble_fraction * sqrt(3.0 / rho_mean);
+            if (ctl->dt_next > minstep) ctl->dt_next = minstep;
+        }
+        ctl->err_old = IH_MAX(err, 1.0e-4);
+        ctl->reject = 0;
+        return 1;
+    }
+    scale = IH_MAX(cfg->safe * pow(err, -alpha), cfg->minscale);
+    (*dt) *= scale;
+    ctl->reject = 1;
+    return 0;
+}
+
+double ih_error_norm(double e5, double e3, double dt, size_t ntot)
+{
+    double deno = e5 + 0.01 * e3;
+    if (deno <= 0.0) deno = 1.0;
+    return dt * e5 * sqrt(1.0 / (ntot * deno));
+}
+
+/* try_step (src/dopri853.c:331-458) + error (:469-486) */
+static int try_step(ih_sim *s, double t, double dt, double *err)
+{
+    const double *k[12];
+    k[0] = s->dfield;
+    for (int j = 1; j < 12; ++j) k[j] = s->stage[j];
+    for (int st = 2; st <= 12; ++st) {
+        int rc = ihg_dp853_stage(s->g, st, dt, s->field, k, s->k_tmp);
+        const double ts = (st == 12) ? t + dt : t + s->dp_c[st - 1] * dt;
+        if (!rc) rc = ih_mk_rhs(s, ts, s->k_tmp, s->stage[st - 1]);
+        if (rc) return rc;
+    }
+    double sums[2];
+    int rc = ihg_dp853_finish(s->g, dt, s->field, k, s->field_new,
+                              s->cfg.rtol, s->cfg.atol, sums);
+    if (rc) return rc;
+    *err = ih_error_norm(sums[0], sums[1], dt, s->Ntot);
+    return 0;
+}
+
+long ih_run_dopri853(ih_sim *s, const ih_dp_opts *opts, ih_save_fn save,
+                     void *user, ih_dp_stats *stats)
+{
+    const ih_config *c = &s->cfg;
+    ih_dp_opts none;
+    memset(&none, 0, sizeof none);
+    if (!opts) opts = &none;
+    if (!s->stage[11]) return IHG_EINVAL; /* created for RK4 only */
+    ih_ctrl ctl = {1.0e-4, c->dt, 0};
+    double dt = c->dt, last_save_t = s->t;
+    long n_stp, n_ok = 0, n_bad = 0, attempts = 0;
+    int rc;
+    const int filter = c->grid.enable_filter ? 1 : 0;
+
+    rc = prepare_and_save(s, save, user);
+    if (rc) return rc;
+    rc = ihg_sync(s->g);
+    if (rc) return rc;
+    const double t_start = wall_time();
+    for (n_stp = 0;; ++n_stp) {
+        if (opts->forced_dt) {
+            if (n_stp >= opts->n_forced) break;
+            dt = opts->forced_dt[n_stp];
+        } else if (s->t + dt * 1.0001 > c->tf) {
+            dt = c->tf - s->t;
+        }
+        /* perform_step, src/dopri853.c:271-311 */
+        double h = dt, err = 0.0, rho_mean = 0.0;
+        int underflow = 0;
+        for (;;) {
+            rc = try_step(s, s->t, h, &err);
+            if (rc) return rc;
+            double means[2];
+            rc = ihg_means(s->g, means); /* stage 12's rho_mean */
+            if (rc) return rc;
+            rho_mean = means[0];
+            const double tried = h;
+            int ok = 1;
+            if (!opts->forced_dt) {
+                ok = ih_success(&ctl, c, err, &h, rho_mean);
+                if (ok && opts->proposer) {
+                    double p = opts->proposer(opts->proposer_user, opts->log,
+                                              IH_MIN(attempts, opts->log_cap),
+                                              ctl.dt_next);
+                    if (p > 0.0 && isfinite(p)) ctl.dt_next = p;
+                }
+            }
+            if (opts->log && attempts < opts->log_cap) {
+                ih_dp_record *r = &opts->log[attempts];
+                r->t = s->t;
+                r->dt = tried;
+                r->err = err;
+                r->rho_mean = rho_mean;
+                r->accepted = ok;
+            }
+            attempts++;
+            if (ok) break;
+            if (fabs(h) <= fabs(s->t) * DBL_EPSILON) {
+                underflow = 1;
+                break;
+            }
+        }
+        if (underflow) break;
+        s->evo_flags.filter = filter;
+        if ((n_stp + 1) % c->skip == 0) s->evo_flags.output = 1;
+        ihg_set_field(s->g, s->field); /* summaries lag: they read `field` */
+        rc = ih_mk_rhs(s, s->t + h, s->field_new, s->dfield_new);
+        const int output = s->evo_flags.output;
+        s->evo_flags.output = 0;
+        s->evo_flags.filter = 0;
+        if (rc) return rc;
+        /* field = field_new, dfield = dfield_new (as a swap) */
+        double *tmp = s->field;
+        s->field = s->field_new;
+        s->field_new = tmp;
+        tmp = s->dfield;
+        s->dfield = s->dfield_new;
+        s->dfield_new = tmp;
+        ihg_set_field(s->g, s->field);
+        s->t += h;
+        if (h == dt)
+            ++n_ok;
+        else
+            ++n_bad;
+        if (output) {
+            rc = do_save(s, save, user, s->t);
+            if (rc) return rc;
+            last_save_t = s->t;
+        }
+        if (s->t >= c->tf) break;
+        if (n_stp >= c->max_steps) break;
+        if (!opts->forced_dt && fabs(ctl.dt_next) <= c->dt_min) break;
+        if (!opts->forced_dt) dt = ctl.dt_next;
+    }
+    rc = ihg_sync(s->g);
+    if (rc) return rc;
+    const double t_end = wall_time();
+    /* wrap_up_dopri853, src/dopri853.c:223-228 */
+    if (last_save_t < s->t) {
+        rc = prepare_and_save(s, save, user);
+        if (rc) return rc;
+    }
+    if (stats) {
+        stats->n_stp = n_stp;
+        stats->n_ok = n_ok;
+        stats->n_bad = n_bad;
+        stats->attempts = attempts;
+        stats->t = s->t;
+        stats->dt_next = ctl.dt_next;
+        stats->err_old = ctl.err_old;
+        stats->integration_seconds = t_end - t_start;
+        stats->calls_rhs = ihg_calls_rhs(s->g);
+    }
+    return attempts;
+}
