@@ -33,7 +33,7 @@ C_FLAGS = ["-std=c99", "-O2", "-fPIC", "-Wall", "-Wextra",
 
 HIP_SOURCES = ["ihg_api.hip", "ihg_fft.hip", "ihg_rhs.hip", "ihg_obs.hip",
                "ihg_step.hip"]
-HOST_SOURCES = ["instaheat.c"]
+HOST_SOURCES = ["instaheat.c", "compat.c"]
 
 
 def _run(cmd):

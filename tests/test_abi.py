@@ -102,3 +102,39 @@ def test_host_error_norm_formula(ih):
         dt * e5 * np.sqrt(1.0 / (n * deno)), rel=1e-15)
     # deno <= 0 falls back to 1
     assert H.ih_error_norm(0.0, 0.0, dt, n) == 0.0
+
+
+def test_compat_layer_exports_the_reference_names(ih):
+    H = ctypes.CDLL(ih.HOST_LIB_PATH)
+    for name in ["mk_rhs", "fft", "ifft", "run_rk4", "run_dopri853",
+                 "evo_flags", "ih_compat_bind", "ih_compat_save_hook",
+                 "ih_compat_status"]:
+        assert hasattr(H, name), name
+
+
+def test_c_host_compiles_against_the_headers(tmp_path, ih):
+    """A C99 host written as in INTEGRATION.md compiles and links."""
+    import subprocess
+    src = tmp_path / "host.c"
+    src.write_text(r"""
+#include <stdio.h>
+#include "instaheat_compat.h"
+int main(void) {
+    ih_config cfg; ih_config_defaults(&cfg);
+    cfg.grid.nx = cfg.grid.ny = cfg.grid.nz = 16;
+    ih_sim *sim = NULL;
+    if (ih_create(&cfg, &sim)) { fputs(ihg_last_error(), stderr); return 1; }
+    ih_compat_bind(sim);
+    evo_flags.filter = 1;
+    run_dopri853();
+    ih_destroy(sim);
+    return ih_compat_status() ? 1 : 0;
+}
+""")
+    libdir = os.path.dirname(ih.HOST_LIB_PATH)
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I",
+                        os.path.join(ROOT, "include"), str(src), "-L", libdir,
+                        "-linstaheat", "-lihg", "-Wl,-rpath," + libdir,
+                        "-o", str(tmp_path / "host")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

