@@ -187,12 +187,14 @@ struct Lds {
 };
 
 // One Stockham stage of radix R on a line in LDS.  Ns = product of the
-// radices of the earlier stages.  `rd` supplies inputs when FIRST (it is
-// called with the point index n), otherwise inputs come from LDS.
-template <int L, int R, int Ns, int DIR, bool FIRST, class Rd>
+// radices of the earlier stages.  When FIRST, inputs come from rd(slot, n)
+// (point n; slot = m * R + r is the register the value lands in) instead of
+// LDS; when LAST_TO_WR, outputs go to wr(slot, n, value) instead of LDS.
+template <int L, int R, int Ns, int DIR, bool FIRST, bool LAST_TO_WR, class Rd,
+          class Wr>
 __device__ __forceinline__ void stockham_stage(double2 *line, int j,
                                                const double2 *__restrict__ tw,
-                                               Rd rd)
+                                               Rd &rd, Wr &wr)
 {
     constexpr int E = Plan<L>::E;
     constexpr int T = L / E;
@@ -206,12 +208,12 @@ __device__ __forceinline__ void stockham_stage(double2 *line, int j,
         for (int r = 0; r < R; ++r) {
             const int n = jb + r * STRIDE_IN;
             if constexpr (FIRST)
-                v[m][r] = rd(n);
+                v[m][r] = rd(m * R + r, n);
             else
                 v[m][r] = line[Lds<L>::idx(n)];
         }
     }
-    if constexpr (!FIRST)
+    if constexpr (!FIRST && !LAST_TO_WR)
         __syncthreads();
 #pragma unroll
     for (int m = 0; m < NB; ++m) {
@@ -231,30 +233,79 @@ __device__ __forceinline__ void stockham_stage(double2 *line, int j,
         const int k = (Ns > 1) ? jb % Ns : 0;
         const int base = (jb / Ns) * Ns * R + k;
 #pragma unroll
-        for (int r = 0; r < R; ++r)
-            line[Lds<L>::idx(base + r * Ns)] = v[m][r];
+        for (int r = 0; r < R; ++r) {
+            if constexpr (LAST_TO_WR)
+                wr(m * R + r, base + r * Ns, v[m][r]);
+            else
+                line[Lds<L>::idx(base + r * Ns)] = v[m][r];
+        }
     }
-    __syncthreads();
+    if constexpr (!LAST_TO_WR)
+        __syncthreads();
 }
 
+// Point index of register slot `slot` (= m * R0 + r) in the first stage of
+// a line transform: n = j + m T + r L / R0.
+template <int L>
+__device__ __forceinline__ int stage1_point(int j, int slot)
+{
+    using P = Plan<L>;
+    constexpr int T = L / P::E;
+    return j + (slot / P::R0) * T + (slot % P::R0) * (L / P::R0);
+}
+
+// Slot permutation between the last stage of one transform and the first
+// stage of the next on the same thread: both hold the same set of points
+// {j + offset}; perm[s'] is the last-stage slot holding the point that
+// first-stage slot s' needs.  Evaluated at compile time.
+template <int L>
+struct SlotPerm {
+    using P = Plan<L>;
+    static constexpr int E = P::E, T = L / P::E;
+    static constexpr int RL = P::R2 > 1 ? P::R2 : (P::R1 > 1 ? P::R1 : P::R0);
+    static constexpr int NSL = L / RL; // Ns of the last stage
+    int perm[E];
+    constexpr SlotPerm() : perm{}
+    {
+        for (int s1 = 0; s1 < E; ++s1) {
+            const int off1 = (s1 / P::R0) * T + (s1 % P::R0) * (L / P::R0);
+            for (int s = 0; s < E; ++s) {
+                const int off = (s / RL) * T + (s % RL) * NSL;
+                if (off == off1) perm[s1] = s;
+            }
+        }
+    }
+};
+
 struct NoRead {
-    __device__ double2 operator()(int) const { return make_double2(0.0, 0.0); }
+    __device__ double2 operator()(int, int) const { return make_double2(0.0, 0.0); }
+};
+struct NoWrite {
+    __device__ void operator()(int, int, double2) const {}
 };
 
 // Full line transform.  On entry either the line is in LDS (FIRST_FROM_RD =
 // false; callers must have synchronised) or `rd` supplies it.  On exit the
-// transformed line is in LDS in natural order and the block is synchronised.
-template <int L, int DIR, bool FIRST_FROM_RD = false, class Rd = NoRead>
+// transformed line is in LDS in natural order and the block is synchronised,
+// unless LAST_TO_WR: then `wr` receives every output point and the LDS line
+// may still be read by other threads of the block (no trailing barrier).
+template <int L, int DIR, bool FIRST_FROM_RD = false, bool LAST_TO_WR = false,
+          class Rd = NoRead, class Wr = NoWrite>
 __device__ __forceinline__ void fft_line(double2 *line, int j,
                                          const double2 *__restrict__ tw,
-                                         Rd rd = Rd())
+                                         Rd rd = Rd(), Wr wr = Wr())
 {
     using P = Plan<L>;
-    stockham_stage<L, P::R0, 1, DIR, FIRST_FROM_RD>(line, j, tw, rd);
+    constexpr bool one = P::R1 == 1;
+    constexpr bool two = P::R1 > 1 && P::R2 == 1;
+    stockham_stage<L, P::R0, 1, DIR, FIRST_FROM_RD, LAST_TO_WR && one>(
+        line, j, tw, rd, wr);
     if constexpr (P::R1 > 1)
-        stockham_stage<L, P::R1, P::R0, DIR, false>(line, j, tw, rd);
+        stockham_stage<L, P::R1, P::R0, DIR, false, LAST_TO_WR && two>(
+            line, j, tw, rd, wr);
     if constexpr (P::R2 > 1)
-        stockham_stage<L, P::R2, P::R0 * P::R1, DIR, false>(line, j, tw, rd);
+        stockham_stage<L, P::R2, P::R0 * P::R1, DIR, false, LAST_TO_WR>(
+            line, j, tw, rd, wr);
 }
 
 } // namespace ihg

@@ -37,7 +37,11 @@ struct LineSet {
 
 // ---------------------------------------------------------------------------
 // strided complex-to-complex pass (y or x), in place or out of place,
-// batched over blockIdx.y (arrays `batch` complex values apart)
+// batched over blockIdx.y (arrays `batch` complex values apart).
+// Lanes run fastest over the B lines of the tile, so for a fixed point n a
+// wavefront touches B consecutive complex values.  The first Stockham stage
+// loads straight from HBM into registers and the last stage stores straight
+// from registers; only the transposes between stages go through LDS.
 // ---------------------------------------------------------------------------
 template <int L, int DIR>
 __global__ __launch_bounds__(FFT_WG) void k_c2c_strided(
@@ -49,26 +53,21 @@ __global__ __launch_bounds__(FFT_WG) void k_c2c_strided(
     const int tid = threadIdx.x;
     in += blockIdx.y * batch;
     out += blockIdx.y * batch;
-    const int b = tid % TL::B, n0 = tid / TL::B;
+    const int b = tid % TL::B, j = tid / TL::B;
     const long line = (long)blockIdx.x * TL::B + b;
     const bool act = line < ls.nlines;
-    const long base = act ? ls.base(line) : 0;
-    double2 *my = lds + b * TL::PITCH;
-#pragma unroll
-    for (int i = 0; i < TL::E; ++i) {
-        const int n = n0 + i * TL::T;
-        my[Lds<L>::idx(n)] =
-            act ? in[base + n * ls.stride] : make_double2(0.0, 0.0);
-    }
-    __syncthreads();
-    fft_line<L, DIR>(lds + (tid / TL::T) * TL::PITCH, tid % TL::T, tw);
-    if (act) {
-#pragma unroll
-        for (int i = 0; i < TL::E; ++i) {
-            const int n = n0 + i * TL::T;
-            out[base + n * ls.stride] = my[Lds<L>::idx(n)];
-        }
-    }
+    const long base = ls.base(act ? line : 0);
+    const double2 *src = in + base;
+    double2 *dst = out + base;
+    const long st = ls.stride;
+    auto rd = [&](int, int n) {
+        const double2 v = src[n * st]; // in bounds for every lane
+        return act ? v : make_double2(0.0, 0.0);
+    };
+    auto wr = [&](int, int n, double2 v) {
+        if (act) dst[n * st] = v;
+    };
+    fft_line<L, DIR, true, true>(lds + b * TL::PITCH, j, tw, rd, wr);
 }
 
 // ---------------------------------------------------------------------------
@@ -226,7 +225,7 @@ __global__ __launch_bounds__(FFT_WG) void k_z_r2c(
     const long line = (long)blockIdx.x * TL::B + bc;
     const bool act = line < nlines;
     const double *ra = in + (act ? 2 * line : 0) * L;
-    auto rd = [&](int n) {
+    auto rd = [&](int, int n) {
         const double u = ra[n], v = ra[L + n]; // in bounds for every lane
         return act ? make_double2(u, v) : make_double2(0.0, 0.0);
     };
@@ -257,7 +256,7 @@ __global__ __launch_bounds__(FFT_WG) void k_z_c2r(
     const long line = (long)blockIdx.x * TL::B + bc;
     const bool act = line < nlines;
     const double2 *A = in + (act ? 2 * line : 0) * MZ;
-    auto rd = [&](int n) {
+    auto rd = [&](int, int n) {
         const double2 z = herm_merge<L>(A, A + MZ, n); // in bounds for every lane
         return act ? z : make_double2(0.0, 0.0);
     };

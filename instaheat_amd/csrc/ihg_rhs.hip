@@ -102,7 +102,7 @@ __global__ __launch_bounds__(FFT_WG) void k_z_c2r_rho(
     const int p = (int)(line & 1);
     const double2 *A = W + (2 * p) * M + row * MZ;
     const double2 *Bq = W + (2 * p + 1) * M + row * MZ;
-    auto rd = [&](int n) {
+    auto rd = [&](int, int n) {
         const double2 z = herm_merge<L>(A, Bq, n); // in bounds for every lane
         return act ? z : make_double2(0.0, 0.0);
     };
@@ -226,7 +226,7 @@ __global__ __launch_bounds__(FFT_WG) void k_z_r2c_sources(
     const double *pb = (q == 0) ? gx : gz;
     const double *pc = (q == 0) ? gx : ((q == 1) ? gy : gz); // S12, S22, S33
     const double *pd = (q == 2) ? gz : gy;
-    auto rd = [&](int n) {
+    auto rd = [&](int, int n) {
         const double u = pa[n] * pb[n], v = pc[n] * pd[n];
         return act ? make_double2(u, v) : make_double2(0.0, 0.0);
     };
