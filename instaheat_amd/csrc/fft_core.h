@@ -254,28 +254,21 @@ __device__ __forceinline__ int stage1_point(int j, int slot)
     return j + (slot / P::R0) * T + (slot % P::R0) * (L / P::R0);
 }
 
-// Slot permutation between the last stage of one transform and the first
-// stage of the next on the same thread: both hold the same set of points
-// {j + offset}; perm[s'] is the last-stage slot holding the point that
-// first-stage slot s' needs.  Evaluated at compile time.
+// Register slots when one transform feeds the next on the same thread.  The
+// last stage of a transform leaves thread j holding points j + T u of the
+// line (u < E), and the first stage of the next transform needs the same
+// points: slot s = m R + r of the last stage (radix RL) holds
+// u = m + r E / RL, first-stage slot s1 = m1 R0 + r1 needs
+// u = m1 + r1 E / R0.  Returns the last-stage slot of first-stage slot s1.
 template <int L>
-struct SlotPerm {
+__device__ __forceinline__ constexpr int chain_slot(int s1)
+{
     using P = Plan<L>;
-    static constexpr int E = P::E, T = L / P::E;
-    static constexpr int RL = P::R2 > 1 ? P::R2 : (P::R1 > 1 ? P::R1 : P::R0);
-    static constexpr int NSL = L / RL; // Ns of the last stage
-    int perm[E];
-    constexpr SlotPerm() : perm{}
-    {
-        for (int s1 = 0; s1 < E; ++s1) {
-            const int off1 = (s1 / P::R0) * T + (s1 % P::R0) * (L / P::R0);
-            for (int s = 0; s < E; ++s) {
-                const int off = (s / RL) * T + (s % RL) * NSL;
-                if (off == off1) perm[s1] = s;
-            }
-        }
-    }
-};
+    constexpr int RL = P::R2 > 1 ? P::R2 : (P::R1 > 1 ? P::R1 : P::R0);
+    constexpr int ML = P::E / RL;
+    const int u = s1 / P::R0 + (s1 % P::R0) * (P::E / P::R0);
+    return (u % ML) * RL + u / ML;
+}
 
 struct NoRead {
     __device__ double2 operator()(int, int) const { return make_double2(0.0, 0.0); }

@@ -71,101 +71,50 @@ __global__ __launch_bounds__(FFT_WG) void k_c2c_strided(
 }
 
 // ---------------------------------------------------------------------------
-// x-axis round trip with a spectral factor: forward x transform, multiply by
-// fac(line, kx_index, value), backward x transform; in place, batched.
-// ---------------------------------------------------------------------------
-template <int L, class Fac>
-__global__ __launch_bounds__(FFT_WG) void k_x_roundtrip(
-    double2 *data, LineSet ls, long batch, const double2 *__restrict__ tw,
-    Fac fac)
-{
-    using TL = Tile<L>;
-    __shared__ double2 lds[TL::B * TL::PITCH];
-    const int tid = threadIdx.x;
-    data += blockIdx.y * batch;
-    const int b = tid % TL::B, n0 = tid / TL::B;
-    const long line = (long)blockIdx.x * TL::B + b;
-    const bool act = line < ls.nlines;
-    const long base = act ? ls.base(line) : 0;
-    double2 *my = lds + b * TL::PITCH;
-    double2 *core = lds + (tid / TL::T) * TL::PITCH;
-    const int j = tid % TL::T;
-#pragma unroll
-    for (int i = 0; i < TL::E; ++i) {
-        const int n = n0 + i * TL::T;
-        my[Lds<L>::idx(n)] =
-            act ? data[base + n * ls.stride] : make_double2(0.0, 0.0);
-    }
-    __syncthreads();
-    fft_line<L, -1>(core, j, tw);
-#pragma unroll
-    for (int i = 0; i < TL::E; ++i) {
-        const int n = n0 + i * TL::T;
-        my[Lds<L>::idx(n)] = fac(blockIdx.y, line, n, my[Lds<L>::idx(n)]);
-    }
-    __syncthreads();
-    fft_line<L, +1>(core, j, tw);
-    if (act) {
-#pragma unroll
-        for (int i = 0; i < TL::E; ++i) {
-            const int n = n0 + i * TL::T;
-            data[base + n * ls.stride] = my[Lds<L>::idx(n)];
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
 // x-axis forward transform followed by NQ backward transforms of the same
-// spectrum, each multiplied by its own factor fac(q, line, kx_index, value)
-// (the spectral derivatives of the right-hand side).  Output q goes to
-// out + q * qstride.
+// spectrum, each multiplied by its own factor fac(q, line, kx_index, value):
+// the spectral derivatives of the right-hand side (NQ = 4, out of place,
+// output q at out + q * qstride) and the filter round trip (NQ = 1, in
+// place, batched over blockIdx.y with arrays `batch` apart).
+// The spectrum never leaves registers: the forward transform's last stage
+// keeps its outputs, and each backward transform's first stage takes them
+// through chain_slot; HBM is read once and written NQ times.
 // ---------------------------------------------------------------------------
 template <int L, int NQ, class Fac>
 __global__ __launch_bounds__(FFT_WG) void k_x_fanout(
-    const double2 *in, double2 *out, long qstride, LineSet ls,
+    const double2 *in, double2 *out, long qstride, LineSet ls, long batch,
     const double2 *__restrict__ tw, Fac fac)
 {
     using TL = Tile<L>;
     __shared__ double2 lds[TL::B * TL::PITCH];
     const int tid = threadIdx.x;
-    const int b = tid % TL::B, n0 = tid / TL::B;
+    in += blockIdx.y * batch;
+    out += blockIdx.y * batch;
+    const int b = tid % TL::B, j = tid / TL::B;
     const long line = (long)blockIdx.x * TL::B + b;
     const bool act = line < ls.nlines;
-    const long base = act ? ls.base(line) : 0;
+    const long base = ls.base(act ? line : 0);
+    const double2 *src = in + base;
+    const long st = ls.stride;
     double2 *my = lds + b * TL::PITCH;
-    double2 *core = lds + (tid / TL::T) * TL::PITCH;
-    const int j = tid % TL::T;
-#pragma unroll
-    for (int i = 0; i < TL::E; ++i) {
-        const int n = n0 + i * TL::T;
-        my[Lds<L>::idx(n)] =
-            act ? in[base + n * ls.stride] : make_double2(0.0, 0.0);
-    }
-    __syncthreads();
-    fft_line<L, -1>(core, j, tw);
-    double2 v[TL::E];
-#pragma unroll
-    for (int i = 0; i < TL::E; ++i)
-        v[i] = my[Lds<L>::idx(n0 + i * TL::T)];
-    __syncthreads();
+    double2 spec[TL::E];
+    auto rd = [&](int, int n) {
+        const double2 v = src[n * st]; // in bounds for every lane
+        return act ? v : make_double2(0.0, 0.0);
+    };
+    auto keep = [&](int slot, int, double2 v) { spec[slot] = v; };
+    fft_line<L, -1, true, true>(my, j, tw, rd, keep);
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-#pragma unroll
-        for (int i = 0; i < TL::E; ++i) {
-            const int n = n0 + i * TL::T;
-            my[Lds<L>::idx(n)] = fac(q, line, n, v[i]);
-        }
-        __syncthreads();
-        fft_line<L, +1>(core, j, tw);
-        if (act) {
-            double2 *o = out + q * qstride;
-#pragma unroll
-            for (int i = 0; i < TL::E; ++i) {
-                const int n = n0 + i * TL::T;
-                o[base + n * ls.stride] = my[Lds<L>::idx(n)];
-            }
-        }
-        __syncthreads();
+        __syncthreads(); // LDS reads of the previous transform are done
+        double2 *dst = out + q * qstride + base;
+        auto rq = [&](int slot, int n) {
+            return fac(q, line, n, spec[chain_slot<L>(slot)]);
+        };
+        auto wq = [&](int, int n, double2 v) {
+            if (act) dst[n * st] = v;
+        };
+        fft_line<L, +1, true, true>(my, j, tw, rq, wq);
     }
 }
 

@@ -201,33 +201,33 @@ __global__ __launch_bounds__(RED_WG) void k_update_phi_psi(
 
 // ---------------------------------------------------------------------------
 // GW sources: z r2c of the six products d_i phi d_j phi (mk_gw_sources,
-// src/toolbox.c:410-424).  blockIdx.y = q picks the pair of products held in
-// one complex row, (S11,S12), (S13,S22) or (S23,S33); component c = 2q+s goes
-// to W + c*M.  q is uniform per work-group, so the loads of the first FFT
-// stage stay free of divergent control flow (with q varying per lane the
-// compiler left the stage-2 twiddle index undefined on one divergent path).
+// src/toolbox.c:410-424).  A work-group owns B rows: every thread loads its
+// first-stage points of d_x phi, d_y phi, d_z phi once into registers, then
+// the three complex rows (S11, S12), (S13, S22), (S23, S33) of each row are
+// transformed one after the other; component c = 2q + s goes to W + c M.
 // ---------------------------------------------------------------------------
-template <int L>
-__global__ __launch_bounds__(FFT_WG) void k_z_r2c_sources(
-    const double *__restrict__ g3, double2 *__restrict__ W, long M,
-    long nrows, long N, const double2 *__restrict__ tw)
+template <int L, int Q, class Rows>
+__device__ __forceinline__ void sources_pair(double2 *lds, int tid,
+                                             const Rows &g, bool act,
+                                             double2 *__restrict__ W, long M,
+                                             long row0, long nrows,
+                                             const double2 *__restrict__ tw)
 {
     using TL = Tile<L>;
     constexpr int MZ = L / 2 + 1;
-    __shared__ double2 lds[TL::B * TL::PITCH];
-    const int tid = threadIdx.x;
     const int bc = tid / TL::T;
-    const int q = blockIdx.y;
-    const long row0 = (long)blockIdx.x * TL::B;
-    const long row = row0 + bc;
-    const bool act = row < nrows;
-    const double *gx = g3 + (act ? row : 0) * L, *gy = gx + N, *gz = gy + N;
-    const double *pa = (q == 2) ? gy : gx; // S11 = x x, S13 = x z, S23 = y z
-    const double *pb = (q == 0) ? gx : gz;
-    const double *pc = (q == 0) ? gx : ((q == 1) ? gy : gz); // S12, S22, S33
-    const double *pd = (q == 2) ? gz : gy;
-    auto rd = [&](int, int n) {
-        const double u = pa[n] * pb[n], v = pc[n] * pd[n];
+    auto rd = [&](int s, int) {
+        double u, v;
+        if constexpr (Q == 0) {
+            u = g.x[s] * g.x[s];
+            v = g.x[s] * g.y[s];
+        } else if constexpr (Q == 1) {
+            u = g.x[s] * g.z[s];
+            v = g.y[s] * g.y[s];
+        } else {
+            u = g.y[s] * g.z[s];
+            v = g.z[s] * g.z[s];
+        }
         return act ? make_double2(u, v) : make_double2(0.0, 0.0);
     };
     fft_line<L, -1, true>(lds + bc * TL::PITCH, tid % TL::T, tw, rd);
@@ -236,10 +236,42 @@ __global__ __launch_bounds__(FFT_WG) void k_z_r2c_sources(
         const int rem = idx - lb * 2 * MZ;
         const int s = rem / MZ, k = rem - s * MZ;
         const long r = row0 + lb;
-        if (r >= nrows) continue;
-        W[(2 * q + s) * M + r * MZ + k] =
-            herm_split<L>(lds + lb * TL::PITCH, k, s);
+        if (r < nrows)
+            W[(2 * Q + s) * M + r * MZ + k] =
+                herm_split<L>(lds + lb * TL::PITCH, k, s);
     }
+    __syncthreads(); // split reads done before the next transform
+}
+
+template <int L>
+struct GradRegs {
+    double x[Plan<L>::E], y[Plan<L>::E], z[Plan<L>::E];
+};
+
+template <int L>
+__global__ __launch_bounds__(FFT_WG) void k_z_r2c_sources(
+    const double *__restrict__ g3, double2 *__restrict__ W, long M,
+    long nrows, long N, const double2 *__restrict__ tw)
+{
+    using TL = Tile<L>;
+    __shared__ double2 lds[TL::B * TL::PITCH];
+    const int tid = threadIdx.x;
+    const int bc = tid / TL::T, j = tid % TL::T;
+    const long row0 = (long)blockIdx.x * TL::B;
+    const long row = row0 + bc;
+    const bool act = row < nrows;
+    const double *gx = g3 + (act ? row : 0) * L;
+    GradRegs<L> g;
+#pragma unroll
+    for (int s = 0; s < TL::E; ++s) {
+        const int n = stage1_point<L>(j, s);
+        g.x[s] = gx[n];
+        g.y[s] = gx[N + n];
+        g.z[s] = gx[2 * N + n];
+    }
+    sources_pair<L, 0>(lds, tid, g, act, W, M, row0, nrows, tw);
+    sources_pair<L, 1>(lds, tid, g, act, W, M, row0, nrows, tw);
+    sources_pair<L, 2>(lds, tid, g, act, W, M, row0, nrows, tw);
 }
 
 __device__ __forceinline__ double2 rc(double a, double2 z)
@@ -348,7 +380,7 @@ int rhs_core(ihg_ctx *c, double *f, double *result, int output)
             hipLaunchKernelGGL((k_x_fanout<L, 4, DerivFac>),
                                dim3((unsigned)((ls.nlines + TL::B - 1) / TL::B)),
                                dim3(FFT_WG), 0, c->stream, W + 4 * M, W, M, ls,
-                               c->tw, fac);
+                               0L, c->tw, fac);
             return hip_check(hipGetLastError(), "k_x_fanout");
         });
     }));
@@ -400,7 +432,7 @@ int rhs_gw(ihg_ctx *c, const double *f, double *result)
         using TL = Tile<L>;
         const unsigned nb = (unsigned)((nrows + TL::B - 1) / TL::B);
         return timed(c, F_Z_R2C_SRC, 3.0 * 8.0 * N + 6.0 * 16.0 * M, [&]() {
-            hipLaunchKernelGGL((k_z_r2c_sources<L>), dim3(nb, 3), dim3(FFT_WG),
+            hipLaunchKernelGGL((k_z_r2c_sources<L>), dim3(nb), dim3(FFT_WG),
                                0, c->stream, c->grad3, W, M, nrows, N, c->tw);
             return hip_check(hipGetLastError(), "k_z_r2c_sources");
         });
@@ -430,10 +462,11 @@ int rhs_filter(ihg_ctx *c, double *f)
         const LineSet ls = x_lineset(c);
         FilterFac fac{c->filter, (long)c->Ny * c->Mz, (double)N};
         return timed(c, F_X_ROUNDTRIP, 4.0 * 2.0 * 16.0 * M, [&]() {
-            hipLaunchKernelGGL((k_x_roundtrip<L, FilterFac>),
+            hipLaunchKernelGGL((k_x_fanout<L, 1, FilterFac>),
                                dim3((unsigned)((ls.nlines + TL::B - 1) / TL::B), 4),
-                               dim3(FFT_WG), 0, c->stream, W, ls, M, c->tw, fac);
-            return hip_check(hipGetLastError(), "k_x_roundtrip");
+                               dim3(FFT_WG), 0, c->stream, W, W, 0L, ls, M,
+                               c->tw, fac);
+            return hip_check(hipGetLastError(), "k_x_fanout (filter)");
         });
     }));
     IHG_TRY(y_pass(c, W, +1, 4, M));
