@@ -167,16 +167,27 @@ __device__ __forceinline__ void dft(double2 *v)
 
 // Factorisation of a line length: E values per thread, radices R0, R1, R2
 // (R1 / R2 = 1 when absent).  Every radix divides E.
+template <int E_, int R0_, int R1_ = 1, int R2_ = 1, int R3_ = 1>
+struct PlanT {
+    static constexpr int E = E_, R0 = R0_, R1 = R1_, R2 = R2_, R3 = R3_;
+};
 template <int L>
 struct Plan;
-template <> struct Plan<4>   { static constexpr int E = 4,  R0 = 4,  R1 = 1,  R2 = 1; };
-template <> struct Plan<8>   { static constexpr int E = 8,  R0 = 8,  R1 = 1,  R2 = 1; };
-template <> struct Plan<16>  { static constexpr int E = 16, R0 = 16, R1 = 1,  R2 = 1; };
-template <> struct Plan<32>  { static constexpr int E = 8,  R0 = 8,  R1 = 4,  R2 = 1; };
-template <> struct Plan<64>  { static constexpr int E = 8,  R0 = 8,  R1 = 8,  R2 = 1; };
-template <> struct Plan<128> { static constexpr int E = 16, R0 = 16, R1 = 8,  R2 = 1; };
-template <> struct Plan<256> { static constexpr int E = 16, R0 = 16, R1 = 16, R2 = 1; };
-template <> struct Plan<512> { static constexpr int E = 16, R0 = 16, R1 = 16, R2 = 2; };
+template <> struct Plan<4>   { static constexpr int E = 4,  R0 = 4,  R1 = 1,  R2 = 1, R3 = 1; };
+template <> struct Plan<8>   { static constexpr int E = 8,  R0 = 8,  R1 = 1,  R2 = 1, R3 = 1; };
+template <> struct Plan<16>  { static constexpr int E = 16, R0 = 16, R1 = 1,  R2 = 1, R3 = 1; };
+template <> struct Plan<32>  { static constexpr int E = 8,  R0 = 8,  R1 = 4,  R2 = 1, R3 = 1; };
+template <> struct Plan<64>  { static constexpr int E = 8,  R0 = 8,  R1 = 8,  R2 = 1, R3 = 1; };
+template <> struct Plan<128> { static constexpr int E = 16, R0 = 16, R1 = 8,  R2 = 1, R3 = 1; };
+template <> struct Plan<256> { static constexpr int E = 16, R0 = 16, R1 = 16, R2 = 1, R3 = 1; };
+template <> struct Plan<512> { static constexpr int E = 16, R0 = 16, R1 = 16, R2 = 2, R3 = 1; };
+
+// radix of the last stage of a plan
+template <class P>
+__host__ __device__ constexpr int last_radix()
+{
+    return P::R3 > 1 ? P::R3 : (P::R2 > 1 ? P::R2 : (P::R1 > 1 ? P::R1 : P::R0));
+}
 
 // LDS geometry of a tile of B lines: one pad slot every 16 points and an odd
 // line pitch, so that both row-wise and column-wise sweeps spread over banks.
@@ -190,13 +201,13 @@ struct Lds {
 // radices of the earlier stages.  When FIRST, inputs come from rd(slot, n)
 // (point n; slot = m * R + r is the register the value lands in) instead of
 // LDS; when LAST_TO_WR, outputs go to wr(slot, n, value) instead of LDS.
-template <int L, int R, int Ns, int DIR, bool FIRST, bool LAST_TO_WR, class Rd,
-          class Wr>
+template <int L, int R, int Ns, int DIR, bool FIRST, bool LAST_TO_WR,
+          class P = Plan<L>, class Rd, class Wr>
 __device__ __forceinline__ void stockham_stage(double2 *line, int j,
                                                const double2 *__restrict__ tw,
                                                Rd &rd, Wr &wr)
 {
-    constexpr int E = Plan<L>::E;
+    constexpr int E = P::E;
     constexpr int T = L / E;
     constexpr int NB = E / R; // butterflies per thread
     constexpr int STRIDE_IN = L / R;
@@ -246,12 +257,22 @@ __device__ __forceinline__ void stockham_stage(double2 *line, int j,
 
 // Point index of register slot `slot` (= m * R0 + r) in the first stage of
 // a line transform: n = j + m T + r L / R0.
-template <int L>
+template <int L, class P = Plan<L>>
 __device__ __forceinline__ int stage1_point(int j, int slot)
 {
-    using P = Plan<L>;
     constexpr int T = L / P::E;
     return j + (slot / P::R0) * T + (slot % P::R0) * (L / P::R0);
+}
+
+// Point index held by register slot `slot` (= m * RL + r) after the last
+// stage (radix RL) of a line transform that ends in registers:
+// n = j + m T + r L / RL.
+template <int L, class P = Plan<L>>
+__device__ __forceinline__ int last_point(int j, int slot)
+{
+    constexpr int RL = last_radix<P>();
+    constexpr int T = L / P::E;
+    return j + (slot / RL) * T + (slot % RL) * (L / RL);
 }
 
 // Register slots when one transform feeds the next on the same thread.  The
@@ -260,11 +281,10 @@ __device__ __forceinline__ int stage1_point(int j, int slot)
 // points: slot s = m R + r of the last stage (radix RL) holds
 // u = m + r E / RL, first-stage slot s1 = m1 R0 + r1 needs
 // u = m1 + r1 E / R0.  Returns the last-stage slot of first-stage slot s1.
-template <int L>
+template <int L, class P = Plan<L>>
 __device__ __forceinline__ constexpr int chain_slot(int s1)
 {
-    using P = Plan<L>;
-    constexpr int RL = P::R2 > 1 ? P::R2 : (P::R1 > 1 ? P::R1 : P::R0);
+    constexpr int RL = last_radix<P>();
     constexpr int ML = P::E / RL;
     const int u = s1 / P::R0 + (s1 % P::R0) * (P::E / P::R0);
     return (u % ML) * RL + u / ML;
@@ -283,22 +303,27 @@ struct NoWrite {
 // unless LAST_TO_WR: then `wr` receives every output point and the LDS line
 // may still be read by other threads of the block (no trailing barrier).
 template <int L, int DIR, bool FIRST_FROM_RD = false, bool LAST_TO_WR = false,
-          class Rd = NoRead, class Wr = NoWrite>
+          class P = Plan<L>, class Rd = NoRead, class Wr = NoWrite>
 __device__ __forceinline__ void fft_line(double2 *line, int j,
                                          const double2 *__restrict__ tw,
                                          Rd rd = Rd(), Wr wr = Wr())
 {
-    using P = Plan<L>;
-    constexpr bool one = P::R1 == 1;
-    constexpr bool two = P::R1 > 1 && P::R2 == 1;
-    stockham_stage<L, P::R0, 1, DIR, FIRST_FROM_RD, LAST_TO_WR && one>(
+    static_assert(P::E % P::R0 == 0 && P::E % P::R1 == 0 &&
+                      P::E % P::R2 == 0 && P::E % P::R3 == 0,
+                  "every radix divides E");
+    static_assert(P::R0 * P::R1 * P::R2 * P::R3 == L, "radices multiply to L");
+    constexpr int NST = 1 + (P::R1 > 1) + (P::R2 > 1) + (P::R3 > 1);
+    stockham_stage<L, P::R0, 1, DIR, FIRST_FROM_RD, LAST_TO_WR && NST == 1, P>(
         line, j, tw, rd, wr);
     if constexpr (P::R1 > 1)
-        stockham_stage<L, P::R1, P::R0, DIR, false, LAST_TO_WR && two>(
+        stockham_stage<L, P::R1, P::R0, DIR, false, LAST_TO_WR && NST == 2, P>(
             line, j, tw, rd, wr);
     if constexpr (P::R2 > 1)
-        stockham_stage<L, P::R2, P::R0 * P::R1, DIR, false, LAST_TO_WR>(
-            line, j, tw, rd, wr);
+        stockham_stage<L, P::R2, P::R0 * P::R1, DIR, false,
+                       LAST_TO_WR && NST == 3, P>(line, j, tw, rd, wr);
+    if constexpr (P::R3 > 1)
+        stockham_stage<L, P::R3, P::R0 * P::R1 * P::R2, DIR, false, LAST_TO_WR,
+                       P>(line, j, tw, rd, wr);
 }
 
 } // namespace ihg

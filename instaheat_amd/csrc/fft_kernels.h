@@ -78,9 +78,11 @@ __global__ __launch_bounds__(FFT_WG) void k_c2c_strided(
 // place, batched over blockIdx.y with arrays `batch` apart).
 // The spectrum never leaves registers: the forward transform's last stage
 // keeps its outputs, and each backward transform's first stage takes them
-// through chain_slot; HBM is read once and written NQ times.
+// through chain_slot; HBM is read once and written NQ times.  With FWD =
+// false the input is already the full spectrum and only the backward
+// transforms run.
 // ---------------------------------------------------------------------------
-template <int L, int NQ, class Fac>
+template <int L, int NQ, bool FWD, class Fac>
 __global__ __launch_bounds__(FFT_WG) void k_x_fanout(
     const double2 *in, double2 *out, long qstride, LineSet ls, long batch,
     const double2 *__restrict__ tw, Fac fac)
@@ -102,14 +104,21 @@ __global__ __launch_bounds__(FFT_WG) void k_x_fanout(
         const double2 v = src[n * st]; // in bounds for every lane
         return act ? v : make_double2(0.0, 0.0);
     };
-    auto keep = [&](int slot, int, double2 v) { spec[slot] = v; };
-    fft_line<L, -1, true, true>(my, j, tw, rd, keep);
+    if constexpr (FWD) {
+        auto keep = [&](int slot, int, double2 v) { spec[slot] = v; };
+        fft_line<L, -1, true, true>(my, j, tw, rd, keep);
+    } else {
+#pragma unroll
+        for (int slot = 0; slot < TL::E; ++slot)
+            spec[slot] = rd(slot, stage1_point<L>(j, slot));
+    }
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-        __syncthreads(); // LDS reads of the previous transform are done
+        if (FWD || q > 0)
+            __syncthreads(); // LDS reads of the previous transform are done
         double2 *dst = out + q * qstride + base;
         auto rq = [&](int slot, int n) {
-            return fac(q, line, n, spec[chain_slot<L>(slot)]);
+            return fac(q, line, n, spec[FWD ? chain_slot<L>(slot) : slot]);
         };
         auto wq = [&](int, int n, double2 v) {
             if (act) dst[n * st] = v;

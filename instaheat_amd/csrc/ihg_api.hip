@@ -387,10 +387,13 @@ extern "C" int ihg_mk_rhs(ihg_ctx *c, double t, double *f, double *result,
     if (f == result) return set_error("f and result must differ");
     c->calls_rhs += 1;
     const int output = (flags & IHG_OUTPUT) ? 1 : 0;
-    if ((flags & IHG_FILTER) && c->p.enable_filter) IHG_TRY(rhs_filter(c, f));
-    IHG_TRY(rhs_core(c, f, result, output));
+    // with the filter, the filtered spectra of phi and psi are kept for the
+    // derivatives and the output (tmp.phic / tmp.psic, src/toolbox.c:164-171)
+    const int kept = (flags & IHG_FILTER) && c->p.enable_filter;
+    if (kept) IHG_TRY(rhs_filter(c, f));
+    IHG_TRY(rhs_core(c, f, result, output, kept));
     if (c->p.enable_gw) IHG_TRY(rhs_gw(c, f, result));
-    if (output) IHG_TRY(rhs_output(c, f));
+    if (output) IHG_TRY(rhs_output(c, f, kept));
     return 0;
 }
 

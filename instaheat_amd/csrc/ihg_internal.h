@@ -119,10 +119,10 @@ int z_c2r_pass(ihg_ctx *c, const double2 *in, double *out, int batch,
                long cbatch, long rbatch);
 
 // right-hand side launchers (ihg_rhs.hip)
-int rhs_core(ihg_ctx *c, double *f, double *result, int output);
+int rhs_core(ihg_ctx *c, double *f, double *result, int output, int kept);
 int rhs_filter(ihg_ctx *c, double *f);
 int rhs_gw(ihg_ctx *c, const double *f, double *result);
-int rhs_output(ihg_ctx *c, const double *f);
+int rhs_output(ihg_ctx *c, const double *f, int kept);
 int initial_psi(ihg_ctx *c, double *field, double *dfield);
 
 int set_error(const char *fmt, ...);

@@ -311,15 +311,16 @@ __global__ __launch_bounds__(RED_WG) void k_neg_ksq(
     }
 }
 
-int rhs_output(ihg_ctx *c, const double *f)
+int rhs_output(ihg_ctx *c, const double *f, int kept)
 {
     const long N = c->N, M = c->M, Ntot = c->Ntot;
     const int bins = c->p.bins;
     const double *field = c->field ? c->field : f;
     double2 *W = c->W, *Wc = c->Wc;
     double *obs = c->dobs;
-    // phi_hat, psi_hat (of the possibly filtered f) and rho_hat
-    IHG_TRY(fft_r2c(c, f, Wc, 2, 2 * N, M));
+    // phi_hat, psi_hat (kept by the filter pass, else of f) and rho_hat
+    if (!kept)
+        IHG_TRY(fft_r2c(c, f, Wc, 2, 2 * N, M));
     IHG_TRY(fft_r2c(c, c->rho, W, 1, 0, 0));
     // power spectra
     {

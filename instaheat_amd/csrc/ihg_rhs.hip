@@ -13,7 +13,7 @@
 //   means           block sums -> rho_mean, pressure_mean, da   (:87, :241)
 //   update          result[0..4N) from f, lap, p, means   (toolbox.c:282-299)
 // GW on adds: z r2c of the six products d_i phi d_j phi (pairs in one complex
-// row), y and x passes, and one pass that does the TT projection and the h
+// row), an x pass, and one y pass that also does the TT projection and the h
 // update (toolbox.c:311-424).
 #include <cfloat>
 #include <cmath>
@@ -63,16 +63,28 @@ struct DerivFac {
     }
 };
 
-// capply_filter (src/toolbox.c:616-626): in *= filter; out = in / N
+// capply_filter (src/toolbox.c:616-626): in *= filter; out = in / N.
+// The filtered spectra in *= filter of phi (batch 0) and psi (batch 2) are
+// kept, as mk_ffts_and_filter keeps tmp.phic / tmp.psic
+// (src/toolbox.c:164-171): phi's in keep_phi (and keep_phi2), psi's in
+// keep_psi; null pointers keep nothing.
 struct FilterFac {
     const double *__restrict__ filt;
     long plane; // Ny * Mz
     double Nd;
+    double2 *keep_phi, *keep_phi2, *keep_psi;
     __device__ double2 operator()(int, long line, int n, double2 v) const
     {
-        // padding lines of the last work-group (line >= plane) read entry 0
-        const double F = filt[(long)n * plane + (line < plane ? line : 0)];
+        const bool in = line < plane; // padding lines read entry 0
+        const long i = (long)n * plane + (in ? line : 0);
+        const double F = filt[i];
         const double2 w = make_double2(v.x * F, v.y * F);
+        if (in && blockIdx.y == 0 && keep_phi) {
+            keep_phi[i] = w;
+            keep_phi2[i] = w;
+        }
+        if (in && blockIdx.y == 2 && keep_psi)
+            keep_psi[i] = w;
         return make_double2(w.x / Nd, w.y / Nd);
     }
 };
@@ -274,18 +286,164 @@ __global__ __launch_bounds__(FFT_WG) void k_z_r2c_sources(
     sources_pair<L, 2>(lds, tid, g, act, W, M, row0, nrows, tw);
 }
 
-__device__ __forceinline__ double2 rc(double a, double2 z)
+// ---------------------------------------------------------------------------
+// TT projection (mk_gw_sources_tt, src/toolbox.c:363-397) as two linear forms
+// of the six source components: o0 = sum_c alpha_c S_c, o1 = sum_c beta_c S_c.
+// With f = k / k^2 (full k), t1 = sum_ij k_i f_j S_ij, t2 = tr S, s1 = t1 + t2,
+// s2 = t1 - t2 and the kept axis pair (p, q) = (x, y) if |kz| > eps, (x, z)
+// if |ky| > eps, (y, z) otherwise, the reference computes
+//   o0 = S_pp - 2 f_p (sum_j k_j S_pj) + (f_p k_p s1 + s2) / 2
+//   o1 = S_pq - f_p (sum_j k_j S_qj) - f_q (sum_j k_j S_pj) + f_p k_q s1 / 2
+// alpha_C, beta_C are the derivatives of these with respect to component
+// C = 0..5 = (xx, xy, xz, yy, yz, zz); the k = 0 mode projects to 0.
+// ---------------------------------------------------------------------------
+template <int C>
+__device__ __forceinline__ void tt_coef(double kx, double ky, double kz,
+                                        double inv_ksq, bool zero, double &al,
+                                        double &be)
 {
-    return make_double2(a * z.x, a * z.y);
+    const double fx = kx * inv_ksq, fy = ky * inv_ksq, fz = kz * inv_ksq;
+    const bool b1 = fabs(kz) > DBL_EPSILON;
+    const bool b2 = !b1 && fabs(ky) > DBL_EPSILON;
+    const bool py = !b1 && !b2; // p = y, else p = x
+    const bool qy = b1;         // q = y, else q = z
+    const double kp = py ? ky : kx, fp = py ? fy : fx;
+    const double kq = qy ? ky : kz, fq = qy ? fy : fz;
+    double t1, t2, rowp, rowq, dpp, dpq;
+    if constexpr (C == 0) {
+        t1 = kx * fx; t2 = 1.0; rowp = py ? 0.0 : kx; rowq = 0.0;
+        dpp = py ? 0.0 : 1.0; dpq = 0.0;
+    } else if constexpr (C == 1) {
+        t1 = 2.0 * (kx * fy); t2 = 0.0; rowp = py ? kx : ky;
+        rowq = qy ? kx : 0.0; dpp = 0.0; dpq = (!py && qy) ? 1.0 : 0.0;
+    } else if constexpr (C == 2) {
+        t1 = 2.0 * (kx * fz); t2 = 0.0; rowp = py ? 0.0 : kz;
+        rowq = qy ? 0.0 : kx; dpp = 0.0; dpq = (!py && !qy) ? 1.0 : 0.0;
+    } else if constexpr (C == 3) {
+        t1 = ky * fy; t2 = 1.0; rowp = py ? ky : 0.0; rowq = qy ? ky : 0.0;
+        dpp = py ? 1.0 : 0.0; dpq = 0.0;
+    } else if constexpr (C == 4) {
+        t1 = 2.0 * (ky * fz); t2 = 0.0; rowp = py ? kz : 0.0;
+        rowq = qy ? kz : ky; dpp = 0.0; dpq = py ? 1.0 : 0.0;
+    } else {
+        t1 = kz * fz; t2 = 1.0; rowp = 0.0; rowq = qy ? 0.0 : kz;
+        dpp = 0.0; dpq = 0.0;
+    }
+    const double s1 = t1 + t2, s2 = t1 - t2;
+    al = zero ? 0.0 : dpp - 2.0 * fp * rowp + 0.5 * (fp * kp * s1 + s2);
+    be = zero ? 0.0 : dpq - fp * rowq - fq * rowp + 0.5 * fp * kq * s1;
 }
 
-// mk_gw_sources_tt (src/toolbox.c:363-397) fused with update_h
-// (src/toolbox.c:311-346)
-__global__ __launch_bounds__(RED_WG) void k_tt_update_h(
-    const double2 *__restrict__ W, long M, const double *__restrict__ f,
-    double *__restrict__ result, long N, long Next, long Ntot, KGeom kg,
-    const double *__restrict__ dscal)
+// y transform of source component C (its z and x passes are done) with the
+// TT accumulation folded into the last Stockham stage; `cur` holds the
+// component's first-stage points (loaded one component ahead)
+template <int L, class P, int C>
+__device__ __forceinline__ void tt_component(
+    double2 *my, int j, const double2 *__restrict__ tw, const double2 *cur,
+    const KGeom &kg, int ix, int iz, const double *inv_ksq, double2 *o0,
+    double2 *o1)
 {
+    const double kx = kg.kxf(ix), kz = kg.kzf(iz);
+    if constexpr (C > 0)
+        __syncthreads(); // LDS reads of the previous component are done
+    // Register pressure control (empty asm statements, no instructions):
+    // an opaque copy of the twiddle pointer per component stops the compiler
+    // from loading every stage twiddle once and holding it across all six
+    // transforms, and pinning the accumulators after each component stops it
+    // from deferring the sums (and so keeping all six spectra live).
+    const double2 *twc = tw;
+    asm volatile("" : "+s"(twc));
+    auto rd = [&](int s, int) { return cur[s]; };
+    auto acc = [&](int s, int n, double2 v) {
+        // recomputed per component from 1/k^2 alone: an empty asm keeps the
+        // compiler from holding every derived coefficient of every point
+        // live across the six transforms
+        double iv = inv_ksq[s];
+        asm volatile("" : "+v"(iv));
+        const bool zero = (ix == 0) && (iz == 0) && (n == 0);
+        double al, be;
+        tt_coef<C>(kx, kg.kyf(n), kz, iv, zero, al, be);
+        if constexpr (C == 0) {
+            o0[s] = make_double2(al * v.x, al * v.y);
+            o1[s] = make_double2(be * v.x, be * v.y);
+        } else {
+            o0[s] = make_double2(o0[s].x + al * v.x, o0[s].y + al * v.y);
+            o1[s] = make_double2(o1[s].x + be * v.x, o1[s].y + be * v.y);
+        }
+    };
+    fft_line<L, -1, true, true, P>(my, j, twc, rd, acc);
+#pragma unroll
+    for (int s = 0; s < P::E; ++s)
+        asm volatile("" : "+v"(o0[s].x), "+v"(o0[s].y), "+v"(o1[s].x),
+                     "+v"(o1[s].y));
+}
+
+// Line plans of the fused y pass: few values per thread (radix-4 stages),
+// so that the two TT accumulators, 1/k^2 and the transform of every point a
+// thread owns stay in registers.
+template <int L> struct TTPlan;
+template <> struct TTPlan<4>   { using P = PlanT<4, 4>; static constexpr int WG = 256; };
+template <> struct TTPlan<8>   { using P = PlanT<4, 4, 2>; static constexpr int WG = 256; };
+template <> struct TTPlan<16>  { using P = PlanT<4, 4, 4>; static constexpr int WG = 256; };
+template <> struct TTPlan<32>  { using P = PlanT<4, 4, 4, 2>; static constexpr int WG = 256; };
+template <> struct TTPlan<64>  { using P = PlanT<4, 4, 4, 4>; static constexpr int WG = 256; };
+template <> struct TTPlan<128> { using P = PlanT<4, 4, 4, 4, 2>; static constexpr int WG = 512; };
+template <> struct TTPlan<256> { using P = PlanT<4, 4, 4, 4, 4>; static constexpr int WG = 512; };
+template <> struct TTPlan<512> { using P = PlanT<8, 8, 4, 4, 4>; static constexpr int WG = 512; };
+
+// Last pass of the GW sources: y transforms of the six components (z and x
+// passes done), the TT projection, and update_h (src/toolbox.c:311-346):
+//   dh = hdot,  dhdot = -(2 p_mean + k^2/a^2) h - 3 H hdot + 2 S^TT / a^2
+template <int L, int WG, class P>
+__global__ __launch_bounds__(WG) void k_y_tt_update_h(
+    const double2 *W, long M, const double *__restrict__ f,
+    double *__restrict__ result, long N, long Next, long Ntot, KGeom kg,
+    const double *__restrict__ dscal, LineSet ls,
+    const double2 *__restrict__ tw)
+{
+    constexpr int E = P::E, T = L / E, B = WG / T, PITCH = Lds<L>::PITCH;
+    __shared__ double2 lds[B * PITCH];
+    const int tid = threadIdx.x, b = tid % B, j = tid / B;
+    const long line = (long)blockIdx.x * B + b;
+    const bool act = line < ls.nlines;
+    const long lc = act ? line : 0;
+    const long base = ls.base(lc);
+    const int ix = (int)(lc / kg.Mz), iz = (int)(lc - (long)ix * kg.Mz);
+    const long st = ls.stride;
+    double2 *my = lds + b * PITCH;
+    double inv[E];
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+        const int n = last_point<L, P>(j, s);
+        const bool zero = (ix == 0) && (iz == 0) && (n == 0);
+        inv[s] = zero ? 0.0 : 1.0 / kg.ksq(ix, n, iz);
+    }
+    // first-stage points of component c, in bounds for every lane; the
+    // loads of component c + 1 are issued before the transform of c (the
+    // barriers wait on LDS only), so their latency hides behind it
+    auto load = [&](int c, double2 *buf) {
+        const double2 *src = W + c * M + base;
+#pragma unroll
+        for (int s = 0; s < E; ++s) {
+            const double2 v = src[(long)stage1_point<L, P>(j, s) * st];
+            buf[s] = act ? v : make_double2(0.0, 0.0);
+        }
+    };
+    double2 o0[E], o1[E], ba[E], bb[E];
+    load(0, ba);
+    load(1, bb);
+    tt_component<L, P, 0>(my, j, tw, ba, kg, ix, iz, inv, o0, o1);
+    load(2, ba);
+    tt_component<L, P, 1>(my, j, tw, bb, kg, ix, iz, inv, o0, o1);
+    load(3, bb);
+    tt_component<L, P, 2>(my, j, tw, ba, kg, ix, iz, inv, o0, o1);
+    load(4, ba);
+    tt_component<L, P, 3>(my, j, tw, bb, kg, ix, iz, inv, o0, o1);
+    load(5, bb);
+    tt_component<L, P, 4>(my, j, tw, ba, kg, ix, iz, inv, o0, o1);
+    tt_component<L, P, 5>(my, j, tw, bb, kg, ix, iz, inv, o0, o1);
+    if (!act)
+        return;
     const double aa = f[Ntot - 1];
     const double a2 = aa * aa;
     const double hubble = sqrt(dscal[S_RHO_MEAN] / 3.0);
@@ -295,59 +453,22 @@ __global__ __launch_bounds__(RED_WG) void k_tt_update_h(
                Ndh2 = Ndh1 + Next;
     const double2 *f2 = reinterpret_cast<const double2 *>(f);
     double2 *r2 = reinterpret_cast<double2 *>(result);
-    const long plane = (long)kg.Ny * kg.Mz;
-    const long stride = (long)gridDim.x * RED_WG;
-    for (long i = (long)blockIdx.x * RED_WG + threadIdx.x; i < M; i += stride) {
-        const int ix = (int)(i / plane);
-        const long rem = i - (long)ix * plane;
-        const int iy = (int)(rem / kg.Mz), iz = (int)(rem - (long)iy * kg.Mz);
-        double2 o0 = make_double2(0.0, 0.0), o1 = make_double2(0.0, 0.0);
-        const double ksq = kg.ksq(ix, iy, iz);
-        if (i != 0) {
-            double2 s[6];
 #pragma unroll
-            for (int cc = 0; cc < 6; ++cc)
-                s[cc] = W[cc * M + i];
-            const double kx = kg.kxf(ix), ky = kg.kyf(iy), kz = kg.kzf(iz);
-            const double fx = kx / ksq, fy = ky / ksq, fz = kz / ksq;
-            const double2 t1 = cadd(
-                cadd(cadd(rc(kx * fx, s[0]), rc(ky * fy, s[3])), rc(kz * fz, s[5])),
-                rc(2.0, cadd(cadd(rc(kx * fy, s[1]), rc(kx * fz, s[2])),
-                             rc(ky * fz, s[4]))));
-            const double2 t2 = cadd(cadd(s[0], s[3]), s[5]);
-            const double2 s1 = cadd(t1, t2), s2 = csub(t1, t2);
-            if (fabs(kz) > DBL_EPSILON) {
-                const double2 k1 = cadd(cadd(rc(kx, s[0]), rc(ky, s[1])), rc(kz, s[2]));
-                const double2 k2 = cadd(cadd(rc(kx, s[1]), rc(ky, s[3])), rc(kz, s[4]));
-                o0 = cadd(csub(s[0], rc(2.0 * fx, k1)),
-                          rc(0.5, cadd(rc(fx * kx, s1), s2)));
-                o1 = cadd(csub(csub(s[1], rc(fx, k2)), rc(fy, k1)),
-                          rc(0.5 * fx * ky, s1));
-            } else if (fabs(ky) > DBL_EPSILON) {
-                const double2 k1 = cadd(cadd(rc(kx, s[0]), rc(ky, s[1])), rc(kz, s[2]));
-                const double2 k3 = cadd(cadd(rc(kx, s[2]), rc(ky, s[4])), rc(kz, s[5]));
-                o0 = cadd(csub(s[0], rc(2.0 * fx, k1)),
-                          rc(0.5, cadd(rc(fx * kx, s1), s2)));
-                o1 = cadd(csub(csub(s[2], rc(fx, k3)), rc(fz, k1)),
-                          rc(0.5 * fx * kz, s1));
-            } else {
-                const double2 k2 = cadd(cadd(rc(kx, s[1]), rc(ky, s[3])), rc(kz, s[4]));
-                const double2 k3 = cadd(cadd(rc(kx, s[2]), rc(ky, s[4])), rc(kz, s[5]));
-                o0 = cadd(csub(s[3], rc(2.0 * fy, k2)),
-                          rc(0.5, cadd(rc(fy * ky, s1), s2)));
-                o1 = cadd(csub(csub(s[4], rc(fy, k3)), rc(fz, k2)),
-                          rc(0.5 * fy * kz, s1));
-            }
-        }
+    for (int s = 0; s < E; ++s) {
+        const int n = last_point<L, P>(j, s);
+        const long i = base + n * st;
+        const double ksq = kg.ksq(ix, n, iz);
         const double2 h1 = f2[Nh1 / 2 + i], h2 = f2[Nh2 / 2 + i];
         const double2 d1 = f2[Ndh1 / 2 + i], d2 = f2[Ndh2 / 2 + i];
         r2[Nh1 / 2 + i] = d1;
         r2[Nh2 / 2 + i] = d2;
         const double tmp = -(2.0 * pm + ksq / a2);
-        r2[Ndh1 / 2 + i] = make_double2(tmp * h1.x - h3 * d1.x + 2.0 * o0.x / a2,
-                                        tmp * h1.y - h3 * d1.y + 2.0 * o0.y / a2);
-        r2[Ndh2 / 2 + i] = make_double2(tmp * h2.x - h3 * d2.x + 2.0 * o1.x / a2,
-                                        tmp * h2.y - h3 * d2.y + 2.0 * o1.y / a2);
+        r2[Ndh1 / 2 + i] =
+            make_double2(tmp * h1.x - h3 * d1.x + 2.0 * o0[s].x / a2,
+                         tmp * h1.y - h3 * d1.y + 2.0 * o0[s].y / a2);
+        r2[Ndh2 / 2 + i] =
+            make_double2(tmp * h2.x - h3 * d2.x + 2.0 * o1[s].x / a2,
+                         tmp * h2.y - h3 * d2.y + 2.0 * o1[s].y / a2);
     }
 }
 
@@ -361,26 +482,34 @@ static LineSet x_lineset(const ihg_ctx *c)
     return LineSet{plane, plane, 0, plane};
 }
 
-int rhs_core(ihg_ctx *c, double *f, double *result, int output)
+int rhs_core(ihg_ctx *c, double *f, double *result, int output, int kept)
 {
     const long N = c->N, M = c->M;
     const double Nd = (double)N;
     const double m2 = c->p.mass * c->p.mass;
     double2 *W = c->W;
-    // phi_hat (partially: z and y passes) into W4
-    IHG_TRY(z_r2c_pass(c, f, W + 4 * M, 1, 0, 0));
-    IHG_TRY(y_pass(c, W + 4 * M, -1, 1, 0));
+    // phi_hat into W4: kept by the filter pass (full spectrum), or here by
+    // the z and y passes (the x pass is the fan-out's first half)
+    if (!kept) {
+        IHG_TRY(z_r2c_pass(c, f, W + 4 * M, 1, 0, 0));
+        IHG_TRY(y_pass(c, W + 4 * M, -1, 1, 0));
+    }
     // x fan-out into the four derivative spectra W0..W3
     IHG_TRY(dispatch_len(c->Nx, [&](auto Lc) {
         constexpr int L = decltype(Lc)::value;
         using TL = Tile<L>;
         const LineSet ls = x_lineset(c);
         DerivFac fac{c->kg, Nd};
+        const dim3 grid((unsigned)((ls.nlines + TL::B - 1) / TL::B));
         return timed(c, F_X_FANOUT, 5.0 * 16.0 * M, [&]() {
-            hipLaunchKernelGGL((k_x_fanout<L, 4, DerivFac>),
-                               dim3((unsigned)((ls.nlines + TL::B - 1) / TL::B)),
-                               dim3(FFT_WG), 0, c->stream, W + 4 * M, W, M, ls,
-                               0L, c->tw, fac);
+            if (kept)
+                hipLaunchKernelGGL((k_x_fanout<L, 4, false, DerivFac>), grid,
+                                   dim3(FFT_WG), 0, c->stream, W + 4 * M, W, M,
+                                   ls, 0L, c->tw, fac);
+            else
+                hipLaunchKernelGGL((k_x_fanout<L, 4, true, DerivFac>), grid,
+                                   dim3(FFT_WG), 0, c->stream, W + 4 * M, W, M,
+                                   ls, 0L, c->tw, fac);
             return hip_check(hipGetLastError(), "k_x_fanout");
         });
     }));
@@ -437,19 +566,30 @@ int rhs_gw(ihg_ctx *c, const double *f, double *result)
             return hip_check(hipGetLastError(), "k_z_r2c_sources");
         });
     }));
-    IHG_TRY(y_pass(c, W, -1, 6, M));
     IHG_TRY(x_pass(c, W, -1, 6, M));
-    // six sources and h1, h2, dh1, dh2 in; the four h blocks of result out
-    return timed(c, F_TT_UPDATE_H, 14.0 * 16.0 * M, [&]() {
-        hipLaunchKernelGGL(k_tt_update_h, dim3(nblocks(M, RED_WG * 2, 65536)),
-                           dim3(RED_WG), 0, c->stream, W, M, f, result, N,
-                           c->Next, c->Ntot, c->kg, c->dscal);
-        return hip_check(hipGetLastError(), "k_tt_update_h");
+    // y pass of the six sources, TT projection and update_h: six sources
+    // and h1, h2, dh1, dh2 in; the four h blocks of result out
+    return dispatch_len(c->Ny, [&](auto Lc) {
+        constexpr int L = decltype(Lc)::value;
+        using TP = TTPlan<L>;
+        constexpr int B = TP::WG / (L / TP::P::E);
+        // lines along y: one per (x, kz); points Mz apart
+        const LineSet ls{(long)c->Nx * c->Mz, c->Mz, (long)c->Ny * c->Mz, c->Mz};
+        const unsigned nb = (unsigned)((ls.nlines + B - 1) / B);
+        return timed(c, F_TT_UPDATE_H, 14.0 * 16.0 * M, [&]() {
+            hipLaunchKernelGGL((k_y_tt_update_h<L, TP::WG, typename TP::P>),
+                               dim3(nb), dim3(TP::WG), 0, c->stream, W, M, f,
+                               result, N, c->Next, c->Ntot, c->kg, c->dscal,
+                               ls, c->tw);
+            return hip_check(hipGetLastError(), "k_y_tt_update_h");
+        });
     });
 }
 
 // mk_ffts_and_filter's filter branch (src/toolbox.c:164-171): phi, dphi,
-// psi, dpsi (contiguous in f) filtered in place, all four in one batch
+// psi, dpsi (contiguous in f) filtered in place, all four in one batch.
+// The filtered spectra of phi (-> W4 for the derivatives and Wc for the
+// output) and psi (-> Wc + M) are kept, as the reference keeps them.
 int rhs_filter(ihg_ctx *c, double *f)
 {
     const long N = c->N, M = c->M;
@@ -460,9 +600,11 @@ int rhs_filter(ihg_ctx *c, double *f)
         constexpr int L = decltype(Lc)::value;
         using TL = Tile<L>;
         const LineSet ls = x_lineset(c);
-        FilterFac fac{c->filter, (long)c->Ny * c->Mz, (double)N};
-        return timed(c, F_X_ROUNDTRIP, 4.0 * 2.0 * 16.0 * M, [&]() {
-            hipLaunchKernelGGL((k_x_fanout<L, 1, FilterFac>),
+        FilterFac fac{c->filter, (long)c->Ny * c->Mz, (double)N, W + 4 * M,
+                      c->Wc, c->Wc + M};
+        // four fields in and out, three kept spectra out
+        return timed(c, F_X_ROUNDTRIP, 11.0 * 16.0 * M, [&]() {
+            hipLaunchKernelGGL((k_x_fanout<L, 1, true, FilterFac>),
                                dim3((unsigned)((ls.nlines + TL::B - 1) / TL::B), 4),
                                dim3(FFT_WG), 0, c->stream, W, W, 0L, ls, M,
                                c->tw, fac);
