@@ -132,6 +132,41 @@ int ihg_prof_read(ihg_ctx *c, double *ms, double *bytes, long *count,
 int ihg_prof_nfam(void);
 const char *ihg_prof_family(int fam);
 
+/* ---- learned DOPRI853 step proposer ------------------------------------
+ * No counterpart in the reference (its controller is success(),
+ * src/dopri853.c:505-540); for hosts that consult a proposer before
+ * success() on accepted steps.  The embedded error estimate still judges
+ * every step. */
+
+/* State features of f: out = {a, variance of phi (normalised by N - 1),
+ * mean of psi^2}.  Synchronises. */
+int ihg_features(ihg_ctx *c, const double *f, double out[3]);
+
+/* A 20-64-64-1 tanh MLP in FP32: inference, gradient and Adam training on
+ * the GPU.  Parameters are one float array of ihg_mlp_nparams() values:
+ * W1[64][20] | b1[64] | W2[64][64] | b2[64] | W3[64] | b3.  x holds n rows
+ * of 20 features, y / t n outputs / targets; host arrays.  The loss is the
+ * mean of (y - t)^2.  An ihg_mlp has its own HIP stream; calls synchronise
+ * it.  ihg_mlp_create takes initial parameters, or NULL for Glorot-uniform
+ * weights from `seed`. */
+#define IHG_MLP_IN 20
+#define IHG_MLP_HIDDEN 64
+typedef struct ihg_mlp ihg_mlp;
+int ihg_mlp_nparams(void);
+int ihg_mlp_create(ihg_mlp **out, const float *params, unsigned long seed);
+int ihg_mlp_destroy(ihg_mlp *m);
+int ihg_mlp_get_params(ihg_mlp *m, float *params);
+int ihg_mlp_set_params(ihg_mlp *m, const float *params);
+int ihg_mlp_forward(ihg_mlp *m, const float *x, float *y, int n);
+/* gradient of the loss over all n samples (deterministic) */
+int ihg_mlp_grad(ihg_mlp *m, const float *x, const float *t, int n,
+                 float *grad, float *loss);
+/* `epochs` passes of Adam (lr, beta 0.9 / 0.999) over shuffled mini-batches
+ * of `batch` samples; epoch_loss[e] = mean loss over epoch e (may be NULL) */
+int ihg_mlp_train(ihg_mlp *m, const float *x, const float *t, int n,
+                  int epochs, int batch, float lr, unsigned long seed,
+                  float *epoch_loss);
+
 /* Debug access to intermediate arrays of the last mk_rhs, copied to host:
  * 0..2 d_x/y/z phi (GW on), 3 rho, 4 pressure, 5 tmp.f (output calls) */
 int ihg_aux(ihg_ctx *c, int which, double *out);

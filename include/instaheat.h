@@ -74,6 +74,8 @@ void ih_set_time(ih_sim *s, double t);
 void ih_set_dt(ih_sim *s, double dt);
 /* MAX_STEPS of the next run */
 void ih_set_max_steps(ih_sim *s, long max_steps);
+/* FINAL_TIME of the next run */
+void ih_set_tf(ih_sim *s, double tf);
 
 /* mk_rhs(t, f, result) with the flags in ih_flags(s) */
 int ih_mk_rhs(ih_sim *s, double t, double *f, double *result);
@@ -90,9 +92,12 @@ int ih_mk_initial_psi(ih_sim *s);
 /* run_rk4 (src/rk4.c:27-132); returns the number of steps or < 0 */
 long ih_run_rk4(ih_sim *s, ih_save_fn save, void *user);
 
-/* one record per attempted DOPRI853 step */
+/* one record per attempted DOPRI853 step; when a log or a proposer is set,
+ * a, var_phi and mean_psi2 are ihg_features() of the state after the step
+ * (of the state before it for rejected attempts) */
 typedef struct ih_dp_record {
     double t, dt, err, rho_mean;
+    double a, var_phi, mean_psi2;
     int accepted;
 } ih_dp_record;
 
@@ -103,9 +108,13 @@ typedef struct ih_dp_stats {
     unsigned long calls_rhs;
 } ih_dp_stats;
 
-/* Optional step proposer consulted before success() on accepted steps:
- * given the classic controller's dt_next it may return another positive
- * proposal (the embedded error still judges every step). */
+/* Optional step proposer, consulted after success() has accepted a step
+ * that did not follow a rejection: history holds the last n_history <= 8
+ * attempts, oldest first, the accepted one last; dt_classic is success()'s
+ * dt_next.  A positive finite return value replaces dt_next, capped by the
+ * same Hubble limit; the embedded error estimate still judges every step,
+ * and rejections fall back to the classic controller. */
+#define IH_HISTORY 8
 typedef double (*ih_dt_proposer)(void *user, const ih_dp_record *history,
                                  long n_history, double dt_classic);
 
@@ -121,6 +130,31 @@ typedef struct ih_dp_opts {
 /* run_dopri853 (src/dopri853.c:112-158) */
 long ih_run_dopri853(ih_sim *s, const ih_dp_opts *opts, ih_save_fn save,
                      void *user, ih_dp_stats *stats);
+
+/* ---- learned step proposer -------------------------------------------
+ * An ih_dt_proposer over libihg's MLP (no counterpart in the reference; the
+ * proposer form of the DOPRI853 controller).  Raw inputs of the newest
+ * record, oldest first over the last IH_HISTORY attempts (the first attempt
+ * repeated while fewer exist):
+ *   x[0..7]   log10 err   (floored at 1e-12)
+ *   x[8..15]  log10 dt
+ *   x[16] log10 a, x[17] log10 H (H = sqrt(rho_mean / 3)),
+ *   x[18] log10 (var phi + 1e-300), x[19] log10 (<psi^2> + 1e-300).
+ * The network sees (x - in_mean) / in_std; its output y gives
+ * log(dt_next / dt) = y * out_std + out_mean, clamped to
+ * [log min_ratio, log max_ratio]. */
+typedef struct ih_mlp_proposer {
+    ihg_mlp *mlp;
+    float in_mean[IHG_MLP_IN], in_std[IHG_MLP_IN];
+    float out_mean, out_std;
+    double min_ratio, max_ratio;
+    long calls; /* proposals made */
+} ih_mlp_proposer;
+void ih_proposer_features(const ih_dp_record *history, long n_history,
+                          double x[IHG_MLP_IN]);
+/* user = ih_mlp_proposer *; returns 0 (keep dt_classic) on any failure */
+double ih_mlp_propose(void *user, const ih_dp_record *history,
+                      long n_history, double dt_classic);
 
 /* success() (src/dopri853.c:505-540) on a controller state, exported so its
  * arithmetic can be checked against the oracle's */

@@ -34,6 +34,7 @@ IH_RK4 = 0
 IH_DOPRI853 = 4
 
 _dptr = ctypes.POINTER(ctypes.c_double)
+_fptr = ctypes.POINTER(ctypes.c_float)
 _vp = ctypes.c_void_p
 
 
@@ -70,7 +71,15 @@ class IhConfig(ctypes.Structure):
 class IhDpRecord(ctypes.Structure):
     _fields_ = [("t", ctypes.c_double), ("dt", ctypes.c_double),
                 ("err", ctypes.c_double), ("rho_mean", ctypes.c_double),
-                ("accepted", ctypes.c_int)]
+                ("a", ctypes.c_double), ("var_phi", ctypes.c_double),
+                ("mean_psi2", ctypes.c_double), ("accepted", ctypes.c_int)]
+
+
+RECORD_FIELDS = ("t", "dt", "err", "rho_mean", "a", "var_phi", "mean_psi2",
+                 "accepted")
+MLP_IN = 20
+MLP_HIDDEN = 64
+IH_HISTORY = 8
 
 
 class IhDpStats(ctypes.Structure):
@@ -93,6 +102,15 @@ class IhDpOpts(ctypes.Structure):
                 ("proposer", PROPOSER_FN), ("proposer_user", _vp)]
 
 
+class IhMlpProposer(ctypes.Structure):
+    _fields_ = [("mlp", _vp),
+                ("in_mean", ctypes.c_float * MLP_IN),
+                ("in_std", ctypes.c_float * MLP_IN),
+                ("out_mean", ctypes.c_float), ("out_std", ctypes.c_float),
+                ("min_ratio", ctypes.c_double), ("max_ratio", ctypes.c_double),
+                ("calls", ctypes.c_long)]
+
+
 class IhCtrl(ctypes.Structure):
     _fields_ = [("err_old", ctypes.c_double), ("dt_next", ctypes.c_double),
                 ("reject", ctypes.c_int)]
@@ -109,14 +127,18 @@ IHG_SYMBOLS = [
     "ihg_means", "ihg_observables", "ihg_calls_rhs", "ihg_mk_initial_psi",
     "ihg_rk4_stage", "ihg_rk4_final", "ihg_dp853_stage", "ihg_dp853_finish",
     "ihg_dp853_tableau", "ihg_aux", "ihg_prof_enable", "ihg_prof_reset",
-    "ihg_prof_read", "ihg_prof_nfam", "ihg_prof_family",
+    "ihg_prof_read", "ihg_prof_nfam", "ihg_prof_family", "ihg_features",
+    "ihg_mlp_nparams", "ihg_mlp_create", "ihg_mlp_destroy",
+    "ihg_mlp_get_params", "ihg_mlp_set_params", "ihg_mlp_forward",
+    "ihg_mlp_grad", "ihg_mlp_train",
 ]
 HOST_SYMBOLS = [
     "ih_config_defaults", "ih_create", "ih_destroy", "ih_gpu", "ih_cfg",
     "ih_field", "ih_dfield", "ih_flags", "ih_time", "ih_set_time", "ih_set_dt",
-    "ih_set_max_steps",
+    "ih_set_max_steps", "ih_set_tf",
     "ih_mk_rhs", "ih_init_gaussian", "ih_mk_initial_psi", "ih_run_rk4",
-    "ih_run_dopri853", "ih_success", "ih_error_norm",
+    "ih_run_dopri853", "ih_success", "ih_error_norm", "ih_proposer_features",
+    "ih_mlp_propose",
 ]
 
 _lib = None
@@ -165,6 +187,16 @@ def lib():
         "ihg_prof_read": (i, [_vp, _dptr, _dptr, ctypes.POINTER(ctypes.c_long), i]),
         "ihg_prof_nfam": (i, []),
         "ihg_prof_family": (ctypes.c_char_p, [i]),
+        "ihg_features": (i, [_vp, _vp, _dptr]),
+        "ihg_mlp_nparams": (i, []),
+        "ihg_mlp_create": (i, [ctypes.POINTER(_vp), _fptr, ctypes.c_ulong]),
+        "ihg_mlp_destroy": (i, [_vp]),
+        "ihg_mlp_get_params": (i, [_vp, _fptr]),
+        "ihg_mlp_set_params": (i, [_vp, _fptr]),
+        "ihg_mlp_forward": (i, [_vp, _fptr, _fptr, i]),
+        "ihg_mlp_grad": (i, [_vp, _fptr, _fptr, i, _fptr, _fptr]),
+        "ihg_mlp_train": (i, [_vp, _fptr, _fptr, i, i, i, ctypes.c_float,
+                              ctypes.c_ulong, _fptr]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -196,6 +228,7 @@ def host_lib():
         "ih_set_time": (None, [_vp, d]),
         "ih_set_dt": (None, [_vp, d]),
         "ih_set_max_steps": (None, [_vp, ctypes.c_long]),
+        "ih_set_tf": (None, [_vp, ctypes.c_double]),
         "ih_cfg": (_vp, [_vp]),
         "ih_flags": (_vp, [_vp]),
         "ih_mk_rhs": (i, [_vp, d, _vp, _vp]),
@@ -207,6 +240,10 @@ def host_lib():
                                             ctypes.POINTER(IhDpStats)]),
         "ih_success": (i, [ctypes.POINTER(IhCtrl), ctypes.POINTER(IhConfig), d,
                            _dptr, d]),
+        "ih_proposer_features": (None, [ctypes.POINTER(IhDpRecord),
+                                        ctypes.c_long, _dptr]),
+        "ih_mlp_propose": (d, [_vp, ctypes.POINTER(IhDpRecord), ctypes.c_long,
+                               d]),
         "ih_error_norm": (d, [d, d, d, ctypes.c_size_t]),
     }
     for name, (res, args) in sig.items():
@@ -381,6 +418,12 @@ class Gpu:
         _check(lib().ihg_aux(self.ctx, which, out.ctypes.data_as(_dptr)))
         return out
 
+    def features(self, f):
+        """{a, variance of phi, mean of psi^2} of the state f (ihg_features)"""
+        out = np.zeros(3)
+        _check(lib().ihg_features(self.ctx, f.ptr, out.ctypes.data_as(_dptr)))
+        return out
+
 
 def prof_enable(ctx, on=True):
     _check(lib().ihg_prof_enable(ctx, int(on)))
@@ -473,6 +516,10 @@ class Sim:
         host_lib().ih_set_max_steps(self.sim, int(n))
         self.cfg.max_steps = int(n)
 
+    def set_tf(self, tf):
+        host_lib().ih_set_tf(self.sim, float(tf))
+        self.cfg.tf = float(tf)
+
     def _dl(self, ptr, n):
         out = np.empty(n)
         _check(lib().ihg_download(self.ctx, out.ctypes.data_as(_vp), ptr, n))
@@ -511,15 +558,25 @@ class Sim:
         return n
 
     def run_dopri853(self, forced=None, saves=None, log_cap=0, proposer=None):
+        """run_dopri853; `proposer` is a Python callable
+        (history records, n, dt_classic) -> dt, or an object with a
+        ``c_proposer()`` method returning (C function, user pointer), such as
+        instaheat_amd.proposer.MlpProposer."""
         fz = None
         if forced is not None:
             fz = np.ascontiguousarray(forced, dtype=np.float64)
         log = (IhDpRecord * log_cap)() if log_cap else None
-        prop = PROPOSER_FN(proposer) if proposer is not None else PROPOSER_FN()
+        user = None
+        if proposer is None:
+            prop = PROPOSER_FN()
+        elif hasattr(proposer, "c_proposer"):
+            prop, user = proposer.c_proposer()
+        else:
+            prop = PROPOSER_FN(lambda u, h, n, dtc: proposer(h, n, dtc))
         opts = IhDpOpts(fz.ctypes.data_as(_dptr) if fz is not None else None,
                         len(fz) if fz is not None else 0,
                         ctypes.cast(log, ctypes.POINTER(IhDpRecord)) if log else None,
-                        log_cap, prop, None)
+                        log_cap, prop, user)
         st = IhDpStats()
 
         def cb(user, t, a, obs, n):
@@ -533,6 +590,6 @@ class Sim:
         out = {k: getattr(st, k) for k, _ in IhDpStats._fields_}
         if log:
             m = min(n, log_cap)
-            out["log"] = [(log[i].t, log[i].dt, log[i].err, log[i].rho_mean,
-                           bool(log[i].accepted)) for i in range(m)]
+            out["log"] = np.array([tuple(getattr(log[i], f) for f in RECORD_FIELDS)
+                                   for i in range(m)], dtype=np.float64).reshape(m, len(RECORD_FIELDS))
         return out

@@ -397,6 +397,13 @@ extern "C" int ihg_mk_rhs(ihg_ctx *c, double t, double *f, double *result,
     return 0;
 }
 
+extern "C" int ihg_features(ihg_ctx *c, const double *f, double out[3])
+{
+    if (!c || !out) return set_error("NULL argument");
+    IHG_TRY(check_ptr(f, "f"));
+    return features(c, f, out);
+}
+
 extern "C" int ihg_means(ihg_ctx *c, double out[2])
 {
     if (!c || !out) return set_error("NULL argument");

@@ -119,7 +119,19 @@ int z_c2r_pass(ihg_ctx *c, const double2 *in, double *out, int batch,
                long cbatch, long rbatch);
 
 // right-hand side launchers (ihg_rhs.hip)
+// scalar slots in ctx->dscal (16 doubles)
+enum {
+    S_RHO_MEAN = 0,
+    S_P_MEAN = 1,
+    S_PHI_MEAN = 2,  // mean of f[0..N) (constraints, psi init)
+    S_DPHI_MEAN = 3, // mean of f[N..2N)
+    S_DDPHI_MEAN = 4,
+    S_FEATURES = 8, // a, variance of phi, mean of psi^2 (step proposer)
+    S_NSCAL = 16
+};
+
 int rhs_core(ihg_ctx *c, double *f, double *result, int output, int kept);
+int features(ihg_ctx *c, const double *f, double out[3]);
 int rhs_filter(ihg_ctx *c, double *f);
 int rhs_gw(ihg_ctx *c, const double *f, double *result);
 int rhs_output(ihg_ctx *c, const double *f, int kept);

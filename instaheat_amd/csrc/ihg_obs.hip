@@ -15,8 +15,7 @@
 
 namespace ihg {
 
-enum { S_RHO_MEAN = 0, S_P_MEAN = 1, S_PHI_MEAN = 2, S_DPHI_MEAN = 3,
-       S_DDPHI_MEAN = 4 };
+
 
 constexpr int MAX_BINS = 2048;
 constexpr unsigned RED_BLOCKS = 512;
@@ -441,6 +440,73 @@ __global__ __launch_bounds__(RED_WG) void k_ipsi_dpsi(
     const long stride = (long)gridDim.x * RED_WG;
     for (long i = (long)blockIdx.x * RED_WG + threadIdx.x; i < N; i += stride)
         field[3 * N + i] = 0.5 * dphi_mean * fm[i] - hubble * field[2 * N + i];
+}
+
+// Step-proposer features (no counterpart in the reference): per-block sums
+// of phi, phi^2 and psi^2
+__global__ __launch_bounds__(RED_WG) void k_features(const double *__restrict__ f,
+                                                     long N,
+                                                     double *__restrict__ part)
+{
+    __shared__ double sh[8];
+    double s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    const long stride = (long)gridDim.x * RED_WG;
+    for (long i = (long)blockIdx.x * RED_WG + threadIdx.x; i < N; i += stride) {
+        const double p = f[i], q = f[2 * N + i];
+        s1 += p;
+        s2 += p * p;
+        s3 += q * q;
+    }
+    s1 = block_reduce<OpSum>(s1, sh, RED_WG);
+    s2 = block_reduce<OpSum>(s2, sh, RED_WG);
+    s3 = block_reduce<OpSum>(s3, sh, RED_WG);
+    if (threadIdx.x == 0) {
+        part[3 * blockIdx.x] = s1;
+        part[3 * blockIdx.x + 1] = s2;
+        part[3 * blockIdx.x + 2] = s3;
+    }
+}
+
+// out = {a, variance of phi, mean of psi^2}
+__global__ __launch_bounds__(RED_WG) void k_features_final(
+    const double *__restrict__ part, int nb, long N, long Ntot,
+    const double *__restrict__ f, double *__restrict__ out)
+{
+    __shared__ double sh[8];
+    double s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    for (int b = threadIdx.x; b < nb; b += RED_WG) {
+        s1 += part[3 * b];
+        s2 += part[3 * b + 1];
+        s3 += part[3 * b + 2];
+    }
+    s1 = block_reduce<OpSum>(s1, sh, RED_WG);
+    s2 = block_reduce<OpSum>(s2, sh, RED_WG);
+    s3 = block_reduce<OpSum>(s3, sh, RED_WG);
+    if (threadIdx.x == 0) {
+        const double var = (s2 - s1 * s1 / (double)N) / (double)(N - 1);
+        out[0] = f[Ntot - 1];
+        out[1] = var > 0.0 ? var : 0.0;
+        out[2] = s3 / (double)N;
+    }
+}
+
+int features(ihg_ctx *c, const double *f, double out[3])
+{
+    const unsigned nb = red_blocks(c->N);
+    hipLaunchKernelGGL(k_features, dim3(nb), dim3(RED_WG), 0, c->stream, f,
+                       c->N, c->partials);
+    IHG_LAUNCHED("k_features");
+    hipLaunchKernelGGL(k_features_final, dim3(1), dim3(RED_WG), 0, c->stream,
+                       c->partials, (int)nb, c->N, c->Ntot, f,
+                       c->dscal + S_FEATURES);
+    IHG_LAUNCHED("k_features_final");
+    IHG_TRY(hip_check(hipMemcpyAsync(c->hscal + S_FEATURES,
+                                     c->dscal + S_FEATURES, 3 * sizeof(double),
+                                     hipMemcpyDeviceToHost, c->stream),
+                      "download features"));
+    IHG_TRY(hip_check(hipStreamSynchronize(c->stream), "sync"));
+    for (int i = 0; i < 3; ++i) out[i] = c->hscal[S_FEATURES + i];
+    return 0;
 }
 
 int initial_psi(ihg_ctx *c, double *field, double *dfield)

@@ -25,16 +25,6 @@
 
 namespace ihg {
 
-// scalar slots in ctx->dscal
-enum {
-    S_RHO_MEAN = 0,
-    S_P_MEAN = 1,
-    S_PHI_MEAN = 2,  // mean of f[0..N) (constraints, psi init)
-    S_DPHI_MEAN = 3, // mean of f[N..2N)
-    S_DDPHI_MEAN = 4,
-    S_NSCAL = 16
-};
-
 static inline unsigned nblocks(long n, int per_block, unsigned cap)
 {
     long b = (n + per_block - 1) / per_block;

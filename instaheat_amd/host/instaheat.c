@@ -136,6 +136,7 @@ double ih_time(const ih_sim *s) { return s->t; }
 void ih_set_time(ih_sim *s, double t) { s->t = t; }
 void ih_set_dt(ih_sim *s, double dt) { s->cfg.dt = dt; }
 void ih_set_max_steps(ih_sim *s, long max_steps) { s->cfg.max_steps = max_steps; }
+void ih_set_tf(ih_sim *s, double tf) { s->cfg.tf = tf; }
 
 int ih_mk_rhs(ih_sim *s, double t, double *f, double *result)
 {
@@ -375,6 +376,15 @@ long ih_run_dopri853(ih_sim *s, const ih_dp_opts *opts, ih_save_fn save,
 
     rc = prepare_and_save(s, save, user);
     if (rc) return rc;
+    /* features for the log and the proposer, starting with the initial state */
+    const int want_feat = opts->log != NULL || opts->proposer != NULL;
+    double feat[3] = {0.0, 0.0, 0.0};
+    ih_dp_record hist[IH_HISTORY];
+    long n_hist = 0;
+    if (want_feat) {
+        rc = ihg_features(s->g, s->field, feat);
+        if (rc) return rc;
+    }
     rc = ihg_sync(s->g);
     if (rc) return rc;
     const double t_start = wall_time();
@@ -396,25 +406,42 @@ long ih_run_dopri853(ih_sim *s, const ih_dp_opts *opts, ih_save_fn save,
             if (rc) return rc;
             rho_mean = means[0];
             const double tried = h;
+            const int after_reject = ctl.reject;
             int ok = 1;
-            if (!opts->forced_dt) {
-                ok = ih_success(&ctl, c, err, &h, rho_mean);
-                if (ok && opts->proposer) {
-                    double p = opts->proposer(opts->proposer_user, opts->log,
-                                              IH_MIN(attempts, opts->log_cap),
-                                              ctl.dt_next);
-                    if (p > 0.0 && isfinite(p)) ctl.dt_next = p;
+            if (!opts->forced_dt) ok = ih_success(&ctl, c, err, &h, rho_mean);
+            if (ok && want_feat) {
+                rc = ihg_features(s->g, s->field_new, feat);
+                if (rc) return rc;
+            }
+            ih_dp_record rec;
+            rec.t = s->t;
+            rec.dt = tried;
+            rec.err = err;
+            rec.rho_mean = rho_mean;
+            rec.a = feat[0];
+            rec.var_phi = feat[1];
+            rec.mean_psi2 = feat[2];
+            rec.accepted = ok;
+            if (opts->log && attempts < opts->log_cap) opts->log[attempts] = rec;
+            /* the last IH_HISTORY attempts, oldest first */
+            if (n_hist == IH_HISTORY) {
+                memmove(hist, hist + 1, (IH_HISTORY - 1) * sizeof hist[0]);
+                n_hist--;
+            }
+            hist[n_hist++] = rec;
+            attempts++;
+            if (ok && opts->proposer && !opts->forced_dt && !after_reject) {
+                double p = opts->proposer(opts->proposer_user, hist, n_hist,
+                                          ctl.dt_next);
+                if (p > 0.0 && isfinite(p)) {
+                    if (c->max_dt_hubble_fraction > 0.0) {
+                        const double cap =
+                            c->max_dt_hubble_fraction * sqrt(3.0 / rho_mean);
+                        if (p > cap) p = cap;
+                    }
+                    ctl.dt_next = p;
                 }
             }
-            if (opts->log && attempts < opts->log_cap) {
-                ih_dp_record *r = &opts->log[attempts];
-                r->t = s->t;
-                r->dt = tried;
-                r->err = err;
-                r->rho_mean = rho_mean;
-                r->accepted = ok;
-            }
-            attempts++;
             if (ok) break;
             if (fabs(h) <= fabs(s->t) * DBL_EPSILON) {
                 underflow = 1;
@@ -473,4 +500,48 @@ long ih_run_dopri853(ih_sim *s, const ih_dp_opts *opts, ih_save_fn save,
         stats->calls_rhs = ihg_calls_rhs(s->g);
     }
     return attempts;
+}
+
+/* ------------------------------------------------------------------------ */
+/* learned step proposer                                                    */
+/* ------------------------------------------------------------------------ */
+
+void ih_proposer_features(const ih_dp_record *history, long n,
+                          double x[IHG_MLP_IN])
+{
+    for (int j = 0; j < IH_HISTORY; ++j) {
+        long i = n - IH_HISTORY + j;
+        if (i < 0) i = 0;
+        const ih_dp_record *r = &history[i];
+        x[j] = log10(IH_MAX(r->err, 1.0e-12));
+        x[IH_HISTORY + j] = log10(r->dt);
+    }
+    const ih_dp_record *r = &history[n - 1];
+    x[16] = log10(r->a);
+    x[17] = log10(sqrt(r->rho_mean / 3.0));
+    x[18] = log10(r->var_phi + 1.0e-300);
+    x[19] = log10(r->mean_psi2 + 1.0e-300);
+}
+
+double ih_mlp_propose(void *user, const ih_dp_record *history, long n,
+                      double dt_classic)
+{
+    (void)dt_classic;
+    ih_mlp_proposer *p = (ih_mlp_proposer *)user;
+    if (!p || !p->mlp || n < 1) return 0.0;
+    double raw[IHG_MLP_IN];
+    float x[IHG_MLP_IN], y;
+    ih_proposer_features(history, n, raw);
+    for (int i = 0; i < IHG_MLP_IN; ++i) {
+        if (!isfinite(raw[i])) return 0.0;
+        x[i] = (float)((raw[i] - p->in_mean[i]) / p->in_std[i]);
+    }
+    if (ihg_mlp_forward(p->mlp, x, &y, 1)) return 0.0;
+    double lr = (double)y * p->out_std + p->out_mean;
+    const double lo = log(p->min_ratio), hi = log(p->max_ratio);
+    if (!isfinite(lr)) return 0.0;
+    if (lr < lo) lr = lo;
+    if (lr > hi) lr = hi;
+    p->calls++;
+    return history[n - 1].dt * exp(lr);
 }

@@ -1,0 +1,136 @@
+"""GPU checks of the learned DOPRI853 step proposer: the FP32 MLP kernels
+(inference, gradient, training) against the CPU oracle of the network, the
+state features against numpy, and what the committed network gains over the
+classic controller, with the accuracy of the tuned form against the FP64
+oracle."""
+import numpy as np
+import pytest
+
+from tests.states import make_state, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ih():
+    import instaheat_amd
+    instaheat_amd.host_lib()
+    return instaheat_amd
+
+
+@pytest.fixture(scope="module")
+def P():
+    from instaheat_amd import proposer
+    return proposer
+
+
+@pytest.fixture(scope="module")
+def M():
+    from oracle import mlp
+    return mlp
+
+
+def _random_problem(M, seed, n):
+    rng = np.random.default_rng(seed)
+    p = rng.normal(0, 0.25, M.NPARAM).astype(np.float32)
+    x = rng.normal(size=(n, M.NI)).astype(np.float32)
+    t = rng.normal(size=n).astype(np.float32)
+    return p, x, t
+
+
+def test_mlp_forward_matches_oracle(P, M):
+    p, x, _ = _random_problem(M, 0, 1000)
+    net = P.Mlp(p)
+    y = net.forward(x)
+    ref = M.forward(p.astype(np.float64), x.astype(np.float64))
+    print("forward rel err", rel_l2(y, ref))
+    assert rel_l2(y, ref) < 1e-5
+    assert np.array_equal(net.params(), p)
+
+
+@pytest.mark.parametrize("n", [64, 1000, 4097])
+def test_mlp_gradient_matches_oracle(P, M, n):
+    p, x, t = _random_problem(M, n, n)
+    net = P.Mlp(p)
+    g, loss = net.grad(x, t)
+    lref, gref = M.loss_grad(p.astype(np.float64), x.astype(np.float64),
+                             t.astype(np.float64))
+    print("n", n, "grad rel err", rel_l2(g, gref), "loss", loss, lref)
+    assert rel_l2(g, gref) < 1e-4
+    assert loss == pytest.approx(lref, rel=1e-5)
+    g2, loss2 = net.grad(x, t)
+    assert np.array_equal(g, g2) and loss == loss2  # deterministic
+
+
+def test_mlp_training_reduces_loss(P, M):
+    rng = np.random.default_rng(5)
+    x = rng.normal(size=(8192, M.NI)).astype(np.float32)
+    w = rng.normal(size=M.NI)
+    t = np.tanh(x @ w / np.sqrt(M.NI)).astype(np.float32)
+    net = P.Mlp(seed=3)
+    hist = net.train(x, t, epochs=20, batch=512, lr=3e-3, seed=7)
+    print("epoch losses", hist[0], hist[-1])
+    assert hist[-1] < 0.1 * hist[0]
+    assert M.loss(net.params().astype(np.float64), x, t) < 0.1 * float(np.var(t))
+
+
+def test_features_match_numpy(ih):
+    shape = (16, 8, 32)
+    g = ih.Gpu(shape, gw=True)
+    y = make_state(shape, seed=4, gw=True, a=1.7)
+    f = g.to_device(y)
+    N = int(np.prod(shape))
+    out = g.features(f)
+    assert out[0] == y[-1]
+    assert out[1] == pytest.approx(np.var(y[:N], ddof=1), rel=1e-8)
+    assert out[2] == pytest.approx(np.mean(y[2 * N:3 * N] ** 2), rel=1e-12)
+
+
+@pytest.mark.parametrize("n,gw", [(32, False), (64, True)])
+def test_learned_proposer_needs_fewer_rhs_evaluations(P, n, gw):
+    """t in [0, 150] at rtol 1e-10: the committed network against success()
+    alone, on seeds the network was not trained on."""
+    prop = P.MlpProposer.load()
+    c = P.compare(prop, n=n, gw=gw, rtol=1e-10, amp=1e-5, seed=21, kmin=0.1,
+                  tf=150.0)
+    p, q = c["plain"], c["tuned"]
+    print("plain", {k: v for k, v in p.items() if k != "state"})
+    print("tuned", {k: v for k, v in q.items() if k != "state"},
+          "ratio", q["calls"] / p["calls"])
+    assert p["sim_time"] == pytest.approx(150.0) and q["sim_time"] == pytest.approx(150.0)
+    assert prop.calls > 0
+    assert q["calls"] < 0.99 * p["calls"]
+    assert q["n_bad"] <= p["n_bad"] + 2
+
+
+def test_tuned_form_accuracy_against_oracle(ih, P, oracle_mod):
+    """End states at t = 40 (32^3, rtol 1e-10) of the plain and the tuned
+    form against the FP64 oracle run at rtol 1e-13 from the same state: the
+    proposer may not make the method less accurate.  (Both forms differ from
+    the tight-tolerance solution by the method's own global error at rtol
+    1e-10, a few 1e-8 here; with the same step sequence the GPU and the
+    oracle agree to 1e-10, see test_gpu_steppers.)"""
+    tf = 40.0
+    prop = P.MlpProposer.load()
+    ends = {}
+    y0 = None
+    for form in ("plain", "tuned"):
+        sim = P.make_sim(32, False, 1e-10, 1e-5, 21, tf)
+        if y0 is None:
+            y0 = sim.field()
+        sim.run_dopri853(proposer=prop if form == "tuned" else None)
+        assert sim.t == pytest.approx(tf, rel=1e-14)
+        ends[form] = sim.field()
+    L = 2 * np.pi / 0.1
+    o = oracle_mod.Oracle(32, 32, 32, bounds=((0.0, L),) * 3, mass=1.0,
+                          gw=False, filt=True, bins=16)
+    yo = y0.copy()
+    ro = o.run_dopri853(yo, np.zeros_like(yo), 0.0, tf, 1e-7, rtol=1e-13,
+                        atol=1e-16, hubble_frac=1e-2, skip=10**9)
+    assert ro["t"] == pytest.approx(tf, rel=1e-14)
+    live = slice(0, 4 * 32 ** 3)
+    err = {k: rel_l2(v[live], yo[live]) for k, v in ends.items()}
+    print("end-state error vs the oracle at rtol 1e-13", err,
+          "tuned vs plain", rel_l2(ends["tuned"][live], ends["plain"][live]))
+    assert err["plain"] < 1e-6
+    assert err["tuned"] < 1.5 * err["plain"] + 1e-9
