@@ -1,4 +1,4 @@
-"""Benchmark of the InstaHeat hot path on one MI355X.
+"""Benchmark of the InstaHeat hot path on one MI355X: tuned vs plain GPU.
 
 Workload: 256^3 FP64, DOPRI853 (rtol 1e-10, atol 1e-14, Hubble cap 1e-2),
 GW tensor source on, spectral filter on, k_min = 0.05 (L = 125.66), m = 1,
@@ -6,10 +6,17 @@ phi0 = 1.0093, dphi0 = -0.7137, a0 = 1, seeded Gaussian fluctuations of
 amplitude 1e-5 (|dphi_k|^2 = 1/(2 a^3 omega_k), cut at N k_min / 4), psi from
 the Einstein constraint, observables every TIME_STEP_SKIPS = 10 steps.
 
-A "step" is one accepted DOPRI853 step (perform_step).  W warm-up steps bring
-the controller from DELTA_T to its Hubble-capped step size; the K timed
-steps run from there.  value = simulated time advanced / wall seconds of the
-K timed steps (the mon.integration span), with every array resident in HBM.
+A "step" is one accepted DOPRI853 step (perform_step).  W warm-up steps of
+the classic controller take the run from DELTA_T through the Hubble-capped
+phase (where every controller takes the capped step) into the phase where
+the error estimate sets the step.  From that state (a device snapshot) each
+form takes 8 untimed steps, filling the step history, then K timed steps:
+  plain  success() alone (the reference's controller);
+  tuned  the learned step proposer (instaheat_amd/proposer_model.json)
+         before success(); the embedded error still judges every step.
+value = the tuned form's simulated time advanced / wall seconds of its K
+timed steps (the mon.integration span), every array resident in HBM; the
+plain form's figures are reported beside it.
 
 Prints ONE JSON line.  --gpus > 1 is refused: this build runs on one GPU.
 """
@@ -34,12 +41,15 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=14)
+    ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--skip", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     return ap.parse_args()
+
+
+HISTORY_STEPS = 8
 
 
 def workload(n):
@@ -50,8 +60,41 @@ def workload(n):
                 amplitude=1e-5, phi0=1.0093, dphi0=-0.7137, seed=1)
 
 
+def timed_form(ih, sim, snap, dt0, prop, steps, profile):
+    """HISTORY_STEPS untimed then `steps` timed accepted steps from snap."""
+    sim.restore(snap)
+    sim.set_dt(dt0)
+    sim.set_max_steps(HISTORY_STEPS - 1)
+    h = sim.run_dopri853(proposer=prop)
+    sim.set_dt(h["dt_next"])
+    sim.set_max_steps(steps - 1)
+    if profile:
+        ih.prof_enable(sim.ctx, True)
+        ih.prof_reset(sim.ctx)
+    t0 = sim.t
+    calls0 = ih.lib().ihg_calls_rhs(sim.ctx)
+    wall0 = time.perf_counter()
+    r = sim.run_dopri853(proposer=prop)
+    wall1 = time.perf_counter()
+    prof = None
+    if profile:
+        prof = ih.prof_read(sim.ctx)
+        ih.prof_enable(sim.ctx, False)
+    calls = ih.lib().ihg_calls_rhs(sim.ctx) - calls0
+    sim_time = sim.t - t0
+    n = r["n_ok"] + r["n_bad"]
+    secs = r["integration_seconds"]
+    return dict(value=sim_time / secs, sim_time=sim_time, secs=secs,
+                wall=wall1 - wall0, steps=n, n_ok=r["n_ok"], n_bad=r["n_bad"],
+                attempts=r["attempts"], calls_rhs=int(calls), prof=prof,
+                t_start=t0, t_end=sim.t, dt=sim_time / max(n, 1),
+                ms_per_step=1e3 * secs / max(n, 1),
+                rhs_per_sim_time=calls / sim_time)
+
+
 def run_gpu(args, wl):
     import instaheat_amd as ih
+    from instaheat_amd.proposer import MlpProposer
     sim = ih.Sim(wl["shape"], bounds=wl["bounds"], mass=wl["mass"], gw=wl["gw"],
                  filt=wl["filt"], bins=wl["bins"], method=ih.IH_DOPRI853,
                  ti=0.0, tf=1e9, dt=1e-7, rtol=1e-10, atol=1e-14,
@@ -63,27 +106,17 @@ def run_gpu(args, wl):
     # warm-up: W accepted steps (the loop takes max_steps + 1 steps)
     sim.set_max_steps(max(args.warmup - 1, 0))
     w = sim.run_dopri853()
-    sim.set_dt(w["dt_next"])
-    sim.set_max_steps(args.steps - 1)
-    ih.prof_enable(sim.ctx, True)
-    ih.prof_reset(sim.ctx)
-    t0 = sim.t
-    calls0 = ih.lib().ihg_calls_rhs(sim.ctx)
-    wall0 = time.perf_counter()
-    r = sim.run_dopri853()
-    wall1 = time.perf_counter()
-    prof = ih.prof_read(sim.ctx)
-    ih.prof_enable(sim.ctx, False)
-    calls = ih.lib().ihg_calls_rhs(sim.ctx) - calls0
-    sim_time = sim.t - t0
-    steps = r["n_ok"] + r["n_bad"]
-    secs = r["integration_seconds"]
-    out = dict(sim_time=sim_time, secs=secs, wall=wall1 - wall0, steps=steps,
-               n_ok=r["n_ok"], n_bad=r["n_bad"], attempts=r["attempts"],
-               calls_rhs=int(calls), prof=prof, t_end=sim.t,
-               warm_t=t0, dt=sim_time / max(steps, 1), y0=y0)
+    snap = sim.snapshot()
+    forms = {}
+    prop = MlpProposer.load()
+    for form in ("plain", "tuned"):
+        forms[form] = timed_form(ih, sim, snap, w["dt_next"],
+                                 prop if form == "tuned" else None,
+                                 args.steps, profile=(form == "plain"))
+    forms["tuned"]["proposals"] = int(prop.calls)
+    sim.drop(snap)
     sim.close()
-    return out
+    return dict(forms=forms, y0=y0, warm_t=w["t"])
 
 
 def roofline(prof):
@@ -144,15 +177,24 @@ def main():
         sys.exit(2)
     wl = workload(args.grid)
     gpu = run_gpu(args, wl)
-    value = gpu["sim_time"] / gpu["secs"]
+    tuned, plain = gpu["forms"]["tuned"], gpu["forms"]["plain"]
+
+    def summary(f):
+        return {"value": round(f["value"], 6), "ms_per_step": round(f["ms_per_step"], 3),
+                "steps_ok": f["n_ok"], "steps_bad": f["n_bad"],
+                "rhs_evals": f["calls_rhs"],
+                "rhs_evals_per_s": round(f["calls_rhs"] / f["secs"], 2),
+                "rhs_evals_per_sim_time": round(f["rhs_per_sim_time"], 3),
+                "mean_dt": f["dt"], "t_start": f["t_start"], "t_end": f["t_end"],
+                "host_wall_s": round(f["wall"], 4)}
     rec = {
         "metric": METRIC,
-        "value": round(value, 6),
+        "value": round(tuned["value"], 6),
         "unit": "sim-time/wall-s",
         "n_gpus": 1,
-        "steps": gpu["steps"],
+        "steps": tuned["steps"],
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * gpu["secs"] / gpu["steps"], 3),
+        "ms_per_step": round(tuned["ms_per_step"], 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -161,19 +203,18 @@ def main():
         "config": {
             "workload": (f"{args.grid}^3 FP64 DOPRI853 rtol 1e-10 atol 1e-14, GW on, "
                          f"filter on, k_min 0.05, m 1, amplitude 1e-5, "
-                         f"TIME_STEP_SKIPS {args.skip}, plain GPU form"),
-            "grid": args.grid, "form": "plain",
-            "rhs_evals": gpu["calls_rhs"],
-            "rhs_evals_per_s": round(gpu["calls_rhs"] / gpu["secs"], 2),
-            "steps_ok": gpu["n_ok"], "steps_bad": gpu["n_bad"],
-            "attempts": gpu["attempts"], "mean_dt": gpu["dt"],
-            "t_start": gpu["warm_t"], "t_end": gpu["t_end"],
-            "host_wall_s": round(gpu["wall"], 4),
+                         f"TIME_STEP_SKIPS {args.skip}; {args.warmup} warm-up "
+                         f"steps to t = {gpu['warm_t']:.2f}, then {HISTORY_STEPS} "
+                         f"untimed + {args.steps} timed steps per form"),
+            "grid": args.grid, "form": "tuned (learned step proposer + success())",
+            "tuned": dict(summary(tuned), proposals=tuned["proposals"]),
+            "plain": summary(plain),
+            "tuned_over_plain": round(tuned["value"] / plain["value"], 4),
         },
-        "roofline": roofline(gpu["prof"]),
+        "roofline": roofline(plain["prof"]),
     }
     if not args.no_cpu_baseline:
-        rec["cpu_baseline"] = cpu_baseline(args, wl, gpu)
+        rec["cpu_baseline"] = cpu_baseline(args, wl, dict(y0=gpu["y0"], dt=plain["dt"]))
     else:
         rec["cpu_baseline"] = None
     print(json.dumps(rec))

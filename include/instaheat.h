@@ -69,6 +69,7 @@ double *ih_field(ih_sim *s);
 double *ih_dfield(ih_sim *s);
 struct ih_evolution_flags *ih_flags(ih_sim *s);
 double ih_time(const ih_sim *s);
+/* sets t and clears the step history the proposer reads */
 void ih_set_time(ih_sim *s, double t);
 /* pars.t.dt: the first step size of the next run (DELTA_T initially) */
 void ih_set_dt(ih_sim *s, double dt);

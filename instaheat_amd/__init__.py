@@ -531,6 +531,25 @@ class Sim:
     def dfield(self):
         return self._dl(host_lib().ih_dfield(self.sim), self.Ntot)
 
+    def snapshot(self):
+        """A device copy of the state and its time (for restore)."""
+        buf = _vp()
+        _check(lib().ihg_malloc(self.ctx, ctypes.byref(buf), self.Ntot))
+        _check(lib().ihg_copy(self.ctx, buf, host_lib().ih_field(self.sim),
+                              self.Ntot))
+        return {"buf": buf, "t": self.t}
+
+    def restore(self, snap):
+        """Back to a snapshot's state and time (clears the step history)."""
+        _check(lib().ihg_copy(self.ctx, host_lib().ih_field(self.sim),
+                              snap["buf"], self.Ntot))
+        self.t = snap["t"]
+
+    def drop(self, snap):
+        if snap.get("buf"):
+            _check(lib().ihg_free(self.ctx, snap["buf"]))
+            snap["buf"] = None
+
     def set_field(self, y):
         y = np.ascontiguousarray(y, dtype=np.float64)
         assert y.size == self.Ntot

@@ -30,6 +30,10 @@ struct ih_sim {
     double *obs;         /* host copy of the observables */
     size_t obs_len;
     double dp_c[13];     /* DOPRI853 nodes c_s */
+    /* the last IH_HISTORY DOPRI853 attempts, oldest first (the step
+       proposer's inputs); kept across runs, cleared by ih_set_time */
+    ih_dp_record hist[IH_HISTORY];
+    long n_hist;
 };
 
 static double wall_time(void)
@@ -133,7 +137,11 @@ double *ih_field(ih_sim *s) { return s->field; }
 double *ih_dfield(ih_sim *s) { return s->dfield; }
 struct ih_evolution_flags *ih_flags(ih_sim *s) { return &s->evo_flags; }
 double ih_time(const ih_sim *s) { return s->t; }
-void ih_set_time(ih_sim *s, double t) { s->t = t; }
+void ih_set_time(ih_sim *s, double t)
+{
+    s->t = t;
+    s->n_hist = 0;
+}
 void ih_set_dt(ih_sim *s, double dt) { s->cfg.dt = dt; }
 void ih_set_max_steps(ih_sim *s, long max_steps) { s->cfg.max_steps = max_steps; }
 void ih_set_tf(ih_sim *s, double tf) { s->cfg.tf = tf; }
@@ -379,8 +387,7 @@ long ih_run_dopri853(ih_sim *s, const ih_dp_opts *opts, ih_save_fn save,
     /* features for the log and the proposer, starting with the initial state */
     const int want_feat = opts->log != NULL || opts->proposer != NULL;
     double feat[3] = {0.0, 0.0, 0.0};
-    ih_dp_record hist[IH_HISTORY];
-    long n_hist = 0;
+    ih_dp_record *hist = s->hist;
     if (want_feat) {
         rc = ihg_features(s->g, s->field, feat);
         if (rc) return rc;
@@ -424,15 +431,15 @@ long ih_run_dopri853(ih_sim *s, const ih_dp_opts *opts, ih_save_fn save,
             rec.accepted = ok;
             if (opts->log && attempts < opts->log_cap) opts->log[attempts] = rec;
             /* the last IH_HISTORY attempts, oldest first */
-            if (n_hist == IH_HISTORY) {
+            if (s->n_hist == IH_HISTORY) {
                 memmove(hist, hist + 1, (IH_HISTORY - 1) * sizeof hist[0]);
-                n_hist--;
+                s->n_hist--;
             }
-            hist[n_hist++] = rec;
+            hist[s->n_hist++] = rec;
             attempts++;
             if (ok && opts->proposer && !opts->forced_dt && !after_reject) {
-                double p = opts->proposer(opts->proposer_user, hist, n_hist,
-                                          ctl.dt_next);
+                double p = opts->proposer(opts->proposer_user, hist,
+                                          s->n_hist, ctl.dt_next);
                 if (p > 0.0 && isfinite(p)) {
                     if (c->max_dt_hubble_fraction > 0.0) {
                         const double cap =
