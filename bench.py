@@ -33,7 +33,7 @@ sys.path.insert(0, HERE)
 
 METRIC = "sim-time/wall-s at 256³ FP64 DOPRI853 rtol 1e-10: tuned vs plain GPU"
 FFT_FAMILIES = ("z_r2c", "z_c2r", "y_pass", "x_pass", "x_fanout", "z_c2r_rho",
-                "z_r2c_sources", "x_roundtrip")
+                "z_r2c_sources", "y_tt_update_h", "x_roundtrip")
 HBM_PEAK_GBS = 8000.0
 
 
@@ -46,7 +46,30 @@ def parse():
     ap.add_argument("--skip", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-probe", type=float, default=0.0, metavar="GB",
+                    help="only time a plain copy kernel of GB gigabytes each "
+                         "way (the HBM ceiling and the counter calibration)")
     return ap.parse_args()
+
+
+def copy_probe(gb, reps=10):
+    """ihg_stream_copy of gb GB -> gb GB, timed by the live profiler."""
+    import ctypes
+    import instaheat_amd as ih
+    g = ih.Gpu((8, 8, 8))
+    n = int(gb * 1e9 / 8) // 2 * 2
+    a, b = g.alloc(n), g.alloc(n)
+    L = ih.lib()
+    for _ in range(2):
+        ih._check(L.ihg_stream_copy(g.ctx, b.ptr, a.ptr, n))
+    ih.prof_enable(g.ctx, True)
+    ih.prof_reset(g.ctx)
+    for _ in range(reps):
+        ih._check(L.ihg_stream_copy(g.ctx, b.ptr, a.ptr, n))
+    ms, by, cnt = ih.prof_read(g.ctx)["stream_copy"]
+    return {"probe": "stream_copy", "bytes_each_way": 8 * n, "launches": cnt,
+            "avg_us": round(1e3 * ms / cnt, 2),
+            "GB_per_s": round(by / (ms / 1e3) / 1e9, 1)}
 
 
 HISTORY_STEPS = 8
@@ -82,6 +105,12 @@ def timed_form(ih, sim, snap, dt0, prop, steps, profile):
         ih.prof_enable(sim.ctx, False)
     calls = ih.lib().ihg_calls_rhs(sim.ctx) - calls0
     sim_time = sim.t - t0
+    print(f"{'tuned' if prop else 'plain'}: history to t = {t0:.4f} (dt_next "
+          f"{h['dt_next']:.4g}), timed {r['n_ok']} + {r['n_bad']} steps to "
+          f"t = {sim.t:.4f} in {r['integration_seconds']:.3f} s, {calls} RHS",
+          file=sys.stderr, flush=True)
+    if not sim_time > 0.0:
+        raise RuntimeError("the timed run advanced no simulated time")
     n = r["n_ok"] + r["n_bad"]
     secs = r["integration_seconds"]
     return dict(value=sim_time / secs, sim_time=sim_time, secs=secs,
@@ -106,6 +135,8 @@ def run_gpu(args, wl):
     # warm-up: W accepted steps (the loop takes max_steps + 1 steps)
     sim.set_max_steps(max(args.warmup - 1, 0))
     w = sim.run_dopri853()
+    print(f"warm-up: {w['n_ok']} + {w['n_bad']} rejected steps to t = {w['t']:.4f}, "
+          f"dt_next {w['dt_next']:.4g}", file=sys.stderr, flush=True)
     snap = sim.snapshot()
     forms = {}
     prop = MlpProposer.load()
@@ -119,7 +150,18 @@ def run_gpu(args, wl):
     return dict(forms=forms, y0=y0, warm_t=w["t"])
 
 
-def roofline(prof):
+def measured_traffic(kernel, grid):
+    """HBM bytes per launch of the dominant kernel from the committed counter
+    summary (two rocprofv3 --pmc passes scaled by a copy kernel), or None."""
+    path = os.path.join(HERE, "profiles", "hbm_traffic_256cube_pmc.json")
+    if grid != 256 or not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        t = json.load(fh)
+    return t["traffic_bytes_per_launch"] if t.get("kernel") == "k_" + kernel else None
+
+
+def roofline(prof, grid=256):
     fams = {k: v for k, v in prof.items() if v[2] > 0}
     dom = max(fams, key=lambda k: fams[k][0])
     ms, by, cnt = fams[dom]
@@ -134,7 +176,8 @@ def roofline(prof):
                for k, v in fams.items()}
     return {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(ach / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic(dom, grid),
             "bytes_per_launch": per_launch, "avg_launch_us": round(avg_s * 1e6, 2),
             "fft_all_passes_GB_per_s": round(fft_by / (fft_ms / 1e3) / 1e9, 1),
             "fft_all_passes_frac": round(fft_by / (fft_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
@@ -175,6 +218,9 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU path is not built", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe > 0:
+        print(json.dumps(copy_probe(args.copy_probe)))
+        return
     wl = workload(args.grid)
     gpu = run_gpu(args, wl)
     tuned, plain = gpu["forms"]["tuned"], gpu["forms"]["plain"]
@@ -211,7 +257,7 @@ def main():
             "plain": summary(plain),
             "tuned_over_plain": round(tuned["value"] / plain["value"], 4),
         },
-        "roofline": roofline(plain["prof"]),
+        "roofline": roofline(plain["prof"], args.grid),
     }
     if not args.no_cpu_baseline:
         rec["cpu_baseline"] = cpu_baseline(args, wl, dict(y0=gpu["y0"], dt=plain["dt"]))

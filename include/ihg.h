@@ -65,6 +65,11 @@ int ihg_upload(ihg_ctx *c, double *dst, const double *src, size_t count);
 int ihg_download(ihg_ctx *c, double *dst, const double *src, size_t count);
 int ihg_copy(ihg_ctx *c, double *dst, const double *src, size_t count);
 int ihg_zero(ihg_ctx *c, double *dst, size_t count);
+/* No counterpart in the reference: dst = src by a plain copy kernel on the
+ * context's stream (16-byte aligned arrays), timed as "stream_copy" by the
+ * live profiler; used for state snapshots and as the HBM ceiling the pass
+ * kernels are compared with. */
+int ihg_stream_copy(ihg_ctx *c, double *dst, const double *src, size_t count);
 int ihg_sync(ihg_ctx *c);
 
 /* fft / ifft (src/toolbox.h:32-33, src/toolbox.c:806-827): 3D r2c and c2r,

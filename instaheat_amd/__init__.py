@@ -123,6 +123,7 @@ SAVE_FN = ctypes.CFUNCTYPE(None, _vp, ctypes.c_double, ctypes.c_double, _dptr,
 IHG_SYMBOLS = [
     "ihg_create", "ihg_destroy", "ihg_sizes", "ihg_last_error", "ihg_malloc",
     "ihg_free", "ihg_upload", "ihg_download", "ihg_copy", "ihg_zero",
+    "ihg_stream_copy",
     "ihg_sync", "ihg_fft_r2c", "ihg_fft_c2r", "ihg_set_field", "ihg_mk_rhs",
     "ihg_means", "ihg_observables", "ihg_calls_rhs", "ihg_mk_initial_psi",
     "ihg_rk4_stage", "ihg_rk4_final", "ihg_dp853_stage", "ihg_dp853_finish",
@@ -166,6 +167,7 @@ def lib():
         "ihg_download": (i, [_vp, _vp, _vp, sz]),
         "ihg_copy": (i, [_vp, _vp, _vp, sz]),
         "ihg_zero": (i, [_vp, _vp, sz]),
+        "ihg_stream_copy": (i, [_vp, _vp, _vp, sz]),
         "ihg_sync": (i, [_vp]),
         "ihg_fft_r2c": (i, [_vp, _vp, _vp]),
         "ihg_fft_c2r": (i, [_vp, _vp, _vp]),
@@ -535,14 +537,14 @@ class Sim:
         """A device copy of the state and its time (for restore)."""
         buf = _vp()
         _check(lib().ihg_malloc(self.ctx, ctypes.byref(buf), self.Ntot))
-        _check(lib().ihg_copy(self.ctx, buf, host_lib().ih_field(self.sim),
-                              self.Ntot))
+        _check(lib().ihg_stream_copy(self.ctx, buf,
+                                     host_lib().ih_field(self.sim), self.Ntot))
         return {"buf": buf, "t": self.t}
 
     def restore(self, snap):
         """Back to a snapshot's state and time (clears the step history)."""
-        _check(lib().ihg_copy(self.ctx, host_lib().ih_field(self.sim),
-                              snap["buf"], self.Ntot))
+        _check(lib().ihg_stream_copy(self.ctx, host_lib().ih_field(self.sim),
+                                     snap["buf"], self.Ntot))
         self.t = snap["t"]
 
     def drop(self, snap):

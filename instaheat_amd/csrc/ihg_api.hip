@@ -5,6 +5,8 @@
 #include <cstdlib>
 #include <cstring>
 
+#include <cstdint>
+
 #include "ihg_internal.h"
 
 namespace ihg {
@@ -73,8 +75,8 @@ static void host_filter_mask(const ihg_ctx *c, double *out)
 
 static const char *const FAM_NAMES[F_NFAM] = {
     "z_r2c", "z_c2r", "y_pass", "x_pass", "x_fanout", "z_c2r_rho", "means",
-    "update_phi_psi", "z_r2c_sources", "tt_update_h", "x_roundtrip",
-    "observables", "dp_stage", "dp_finish", "rk4"};
+    "update_phi_psi", "z_r2c_sources", "y_tt_update_h", "x_roundtrip",
+    "observables", "dp_stage", "dp_finish", "rk4", "stream_copy"};
 
 static int prof_flush(ihg_ctx *c)
 {
@@ -338,6 +340,36 @@ extern "C" int ihg_copy(ihg_ctx *c, double *dst, const double *src,
     return hip_check(hipMemcpyAsync(dst, src, count * sizeof(double),
                                     hipMemcpyDeviceToDevice, c->stream),
                      "hipMemcpyAsync D2D");
+}
+
+// a plain copy kernel (one double2 per thread; an odd last element by
+// thread 0 of block 0): state snapshots, and the HBM ceiling and counter
+// calibration the pass figures are compared with
+__global__ __launch_bounds__(256) void k_stream_copy(
+    const double *__restrict__ src, double *__restrict__ dst, long count)
+{
+    const long n2 = count / 2;
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n2)
+        reinterpret_cast<double2 *>(dst)[i] =
+            reinterpret_cast<const double2 *>(src)[i];
+    if (i == 0 && (count & 1))
+        dst[count - 1] = src[count - 1];
+}
+
+extern "C" int ihg_stream_copy(ihg_ctx *c, double *dst, const double *src,
+                               size_t count)
+{
+    if (!c || !dst || !src) return set_error("NULL argument");
+    if (((uintptr_t)dst | (uintptr_t)src) % 16)
+        return set_error("ihg_stream_copy: 16-byte aligned arrays only");
+    if (count == 0) return 0;
+    const long n2 = (long)(count / 2) > 0 ? (long)(count / 2) : 1;
+    return timed(c, F_STREAM_COPY, 16.0 * count, [&]() {
+        hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)((n2 + 255) / 256)),
+                           dim3(256), 0, c->stream, src, dst, (long)count);
+        return hip_check(hipGetLastError(), "k_stream_copy");
+    });
 }
 
 extern "C" int ihg_zero(ihg_ctx *c, double *dst, size_t count)
