@@ -14,6 +14,8 @@ form takes 8 untimed steps, filling the step history, then K timed steps:
   plain  success() alone (the reference's controller);
   tuned  the learned step proposer (instaheat_amd/proposer_model.json)
          before success(); the embedded error still judges every step.
+A third form, precision (the GW source and spectrum in FP32), replays the
+plain form's steps and reports its GW spectrum error per k bin.
 value = the tuned form's simulated time advanced / wall seconds of its K
 timed steps (the mon.integration span), every array resident in HBM; the
 plain form's figures are reported beside it.
@@ -83,12 +85,14 @@ def workload(n):
                 amplitude=1e-5, phi0=1.0093, dphi0=-0.7137, seed=1)
 
 
-def timed_form(ih, sim, snap, dt0, prop, steps, profile):
-    """HISTORY_STEPS untimed then `steps` timed accepted steps from snap."""
+def timed_form(ih, sim, snap, dt0, prop, steps, profile, forced=None, tag=""):
+    """HISTORY_STEPS untimed then `steps` timed accepted steps from snap;
+    forced = (history dts, timed dts) replays another form's steps."""
     sim.restore(snap)
     sim.set_dt(dt0)
     sim.set_max_steps(HISTORY_STEPS - 1)
-    h = sim.run_dopri853(proposer=prop)
+    h = sim.run_dopri853(proposer=prop, log_cap=1000,
+                         forced=None if forced is None else forced[0])
     sim.set_dt(h["dt_next"])
     sim.set_max_steps(steps - 1)
     if profile:
@@ -96,8 +100,10 @@ def timed_form(ih, sim, snap, dt0, prop, steps, profile):
         ih.prof_reset(sim.ctx)
     t0 = sim.t
     calls0 = ih.lib().ihg_calls_rhs(sim.ctx)
+    saves = []
     wall0 = time.perf_counter()
-    r = sim.run_dopri853(proposer=prop)
+    r = sim.run_dopri853(proposer=prop, log_cap=1000, saves=saves,
+                         forced=None if forced is None else forced[1])
     wall1 = time.perf_counter()
     prof = None
     if profile:
@@ -105,20 +111,25 @@ def timed_form(ih, sim, snap, dt0, prop, steps, profile):
         ih.prof_enable(sim.ctx, False)
     calls = ih.lib().ihg_calls_rhs(sim.ctx) - calls0
     sim_time = sim.t - t0
-    print(f"{'tuned' if prop else 'plain'}: history to t = {t0:.4f} (dt_next "
-          f"{h['dt_next']:.4g}), timed {r['n_ok']} + {r['n_bad']} steps to "
-          f"t = {sim.t:.4f} in {r['integration_seconds']:.3f} s, {calls} RHS",
+    print(f"{tag}: history to t = {t0:.4f} (dt_next {h['dt_next']:.4g}), timed "
+          f"{r['n_ok']} + {r['n_bad']} steps to t = {sim.t:.4f} in "
+          f"{r['integration_seconds']:.3f} s, {calls} RHS",
           file=sys.stderr, flush=True)
     if not sim_time > 0.0:
         raise RuntimeError("the timed run advanced no simulated time")
     n = r["n_ok"] + r["n_bad"]
     secs = r["integration_seconds"]
+
+    def accepted(log):
+        return log[log[:, 7] > 0.5, 1]
     return dict(value=sim_time / secs, sim_time=sim_time, secs=secs,
                 wall=wall1 - wall0, steps=n, n_ok=r["n_ok"], n_bad=r["n_bad"],
                 attempts=r["attempts"], calls_rhs=int(calls), prof=prof,
                 t_start=t0, t_end=sim.t, dt=sim_time / max(n, 1),
                 ms_per_step=1e3 * secs / max(n, 1),
-                rhs_per_sim_time=calls / sim_time)
+                rhs_per_sim_time=calls / sim_time,
+                dts=(accepted(h["log"]), accepted(r["log"])),
+                last_obs=saves[-1][2] if saves else None)
 
 
 def run_gpu(args, wl):
@@ -143,8 +154,20 @@ def run_gpu(args, wl):
     for form in ("plain", "tuned"):
         forms[form] = timed_form(ih, sim, snap, w["dt_next"],
                                  prop if form == "tuned" else None,
-                                 args.steps, profile=(form == "plain"))
+                                 args.steps, profile=(form == "plain"), tag=form)
     forms["tuned"]["proposals"] = int(prop.calls)
+    # the precision form (FP32 GW source and spectrum) over the plain form's
+    # steps, its GW spectrum per k bin against the plain form's
+    ih._check(ih.lib().ihg_set_gw_fp32(sim.ctx, 1))
+    forms["precision"] = timed_form(ih, sim, snap, w["dt_next"], None,
+                                    args.steps, profile=False,
+                                    forced=forms["plain"]["dts"],
+                                    tag="precision")
+    ih._check(ih.lib().ihg_set_gw_fp32(sim.ctx, 0))
+    g64 = ih.unpack_obs(forms["plain"]["last_obs"], wl["bins"])["gw"]
+    g32 = ih.unpack_obs(forms["precision"]["last_obs"], wl["bins"])["gw"]
+    live = g64 > 1e-6 * g64.max()
+    forms["precision"]["gw_rel_err"] = (np.abs(g32 - g64)[live] / g64[live]).tolist()
     sim.drop(snap)
     sim.close()
     return dict(forms=forms, y0=y0, warm_t=w["t"])
@@ -256,6 +279,12 @@ def main():
             "tuned": dict(summary(tuned), proposals=tuned["proposals"]),
             "plain": summary(plain),
             "tuned_over_plain": round(tuned["value"] / plain["value"], 4),
+            "precision": dict(
+                summary(gpu["forms"]["precision"]),
+                form="FP32 GW source + spectrum over the plain form's steps",
+                gw_spectrum_rel_err_max=max(gpu["forms"]["precision"]["gw_rel_err"]),
+                gw_spectrum_rel_err_median=float(np.median(gpu["forms"]["precision"]["gw_rel_err"])),
+                gw_bins_compared=len(gpu["forms"]["precision"]["gw_rel_err"])),
         },
         "roofline": roofline(plain["prof"], args.grid),
     }

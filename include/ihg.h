@@ -44,6 +44,10 @@ typedef struct ihg_params {
     int enable_gw;         /* ENABLE_GW */
     int enable_filter;     /* ENABLE_FFT_FILTER */
     int bins;              /* POWER_SPECTRUM_BINS */
+    int gw_fp32;           /* precision form, no counterpart in the reference:
+                              1 = GW source (products, FFTs, TT projection)
+                              and GW spectrum in FP32; the state, h update and
+                              everything else stay FP64 */
 } ihg_params;
 
 typedef struct ihg_ctx ihg_ctx;
@@ -137,6 +141,9 @@ int ihg_prof_read(ihg_ctx *c, double *ms, double *bytes, long *count,
 int ihg_prof_nfam(void);
 const char *ihg_prof_family(int fam);
 
+/* Switch the GW precision form (ihg_params.gw_fp32) for later calls. */
+int ihg_set_gw_fp32(ihg_ctx *c, int on);
+
 /* ---- learned DOPRI853 step proposer ------------------------------------
  * No counterpart in the reference (its controller is success(),
  * src/dopri853.c:505-540); for hosts that consult a proposer before
@@ -173,7 +180,8 @@ int ihg_mlp_train(ihg_mlp *m, const float *x, const float *t, int n,
                   float *epoch_loss);
 
 /* Debug access to intermediate arrays of the last mk_rhs, copied to host:
- * 0..2 d_x/y/z phi (GW on), 3 rho, 4 pressure, 5 tmp.f (output calls) */
+ * 0..2 d_x/y/z phi (GW on; FP32 values with gw_fp32), 3 rho, 4 pressure,
+ * 5 tmp.f (output calls) */
 int ihg_aux(ihg_ctx *c, int which, double *out);
 
 #ifdef __cplusplus

@@ -50,7 +50,7 @@ class IhgParams(ctypes.Structure):
         ("za", ctypes.c_double), ("zb", ctypes.c_double),
         ("mass", ctypes.c_double),
         ("enable_gw", ctypes.c_int), ("enable_filter", ctypes.c_int),
-        ("bins", ctypes.c_int),
+        ("bins", ctypes.c_int), ("gw_fp32", ctypes.c_int),
     ]
 
 
@@ -129,6 +129,7 @@ IHG_SYMBOLS = [
     "ihg_rk4_stage", "ihg_rk4_final", "ihg_dp853_stage", "ihg_dp853_finish",
     "ihg_dp853_tableau", "ihg_aux", "ihg_prof_enable", "ihg_prof_reset",
     "ihg_prof_read", "ihg_prof_nfam", "ihg_prof_family", "ihg_features",
+    "ihg_set_gw_fp32",
     "ihg_mlp_nparams", "ihg_mlp_create", "ihg_mlp_destroy",
     "ihg_mlp_get_params", "ihg_mlp_set_params", "ihg_mlp_forward",
     "ihg_mlp_grad", "ihg_mlp_train",
@@ -190,6 +191,7 @@ def lib():
         "ihg_prof_nfam": (i, []),
         "ihg_prof_family": (ctypes.c_char_p, [i]),
         "ihg_features": (i, [_vp, _vp, _dptr]),
+        "ihg_set_gw_fp32": (i, [_vp, i]),
         "ihg_mlp_nparams": (i, []),
         "ihg_mlp_create": (i, [ctypes.POINTER(_vp), _fptr, ctypes.c_ulong]),
         "ihg_mlp_destroy": (i, [_vp]),
@@ -282,11 +284,11 @@ def unpack_obs(obs, bins):
 
 
 def make_params(shape, bounds=((0.0, 1.0),) * 3, mass=1.0, gw=False,
-                filt=True, bins=50):
+                filt=True, bins=50, gw_fp32=False):
     nx, ny, nz = shape
     return IhgParams(nx, ny, nz, bounds[0][0], bounds[0][1], bounds[1][0],
                      bounds[1][1], bounds[2][0], bounds[2][1], mass, int(gw),
-                     int(filt), bins)
+                     int(filt), bins, int(gw_fp32))
 
 
 class DeviceArray:
@@ -328,8 +330,8 @@ class Gpu:
     """One libihg context: the reference's globals for one parameter set."""
 
     def __init__(self, shape, bounds=((0.0, 1.0),) * 3, mass=1.0, gw=False,
-                 filt=True, bins=50):
-        self.params = make_params(shape, bounds, mass, gw, filt, bins)
+                 filt=True, bins=50, gw_fp32=False):
+        self.params = make_params(shape, bounds, mass, gw, filt, bins, gw_fp32)
         self.bins = bins
         c = _vp()
         _check(lib().ihg_create(ctypes.byref(self.params), ctypes.byref(c)))
@@ -474,9 +476,10 @@ class Sim:
                  filt=True, bins=50, method=IH_DOPRI853, ti=0.0, tf=1.0,
                  dt=1e-7, dt_min=1e-9, max_steps=10**15, rtol=1e-10,
                  atol=1e-14, hubble_frac=1e-2, skip=1, a_initial=1.0,
-                 beta=0.0, safe=0.9, minscale=0.333, maxscale=3.0):
+                 beta=0.0, safe=0.9, minscale=0.333, maxscale=3.0,
+                 gw_fp32=False):
         cfg = default_config()
-        cfg.grid = make_params(shape, bounds, mass, gw, filt, bins)
+        cfg.grid = make_params(shape, bounds, mass, gw, filt, bins, gw_fp32)
         cfg.integration_method = method
         cfg.ti, cfg.tf, cfg.dt, cfg.dt_min = ti, tf, dt, dt_min
         cfg.max_steps = min(int(max_steps), 2**62)

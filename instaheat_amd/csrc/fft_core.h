@@ -16,40 +16,58 @@
 
 namespace ihg {
 
-__device__ __forceinline__ double2 cadd(double2 a, double2 b)
+// complex value types: double2 (FP64 path) and float2 (the FP32 GW source)
+template <class C> struct CT;
+template <> struct CT<double2> {
+    using R = double;
+    __device__ static __forceinline__ double2 make(double a, double b)
+    {
+        return make_double2(a, b);
+    }
+};
+template <> struct CT<float2> {
+    using R = float;
+    __device__ static __forceinline__ float2 make(float a, float b)
+    {
+        return make_float2(a, b);
+    }
+};
+
+template <class C>
+__device__ __forceinline__ C cadd(C a, C b)
 {
-    return make_double2(a.x + b.x, a.y + b.y);
+    return CT<C>::make(a.x + b.x, a.y + b.y);
 }
-__device__ __forceinline__ double2 csub(double2 a, double2 b)
+template <class C>
+__device__ __forceinline__ C csub(C a, C b)
 {
-    return make_double2(a.x - b.x, a.y - b.y);
+    return CT<C>::make(a.x - b.x, a.y - b.y);
 }
-__device__ __forceinline__ double2 cmul(double2 a, double2 b)
+template <class C>
+__device__ __forceinline__ C cmul(C a, C b)
 {
-    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    return CT<C>::make(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
-__device__ __forceinline__ double2 cconj(double2 a)
+template <class C>
+__device__ __forceinline__ C cconj(C a)
 {
-    return make_double2(a.x, -a.y);
-}
-__device__ __forceinline__ double2 cscale(double2 a, double s)
-{
-    return make_double2(a.x * s, a.y * s);
+    return CT<C>::make(a.x, -a.y);
 }
 // a * (DIR * i)
-template <int DIR>
-__device__ __forceinline__ double2 mul_i(double2 a)
+template <int DIR, class C>
+__device__ __forceinline__ C mul_i(C a)
 {
     if constexpr (DIR < 0)
-        return make_double2(a.y, -a.x);
+        return CT<C>::make(a.y, -a.x);
     else
-        return make_double2(-a.y, a.x);
+        return CT<C>::make(-a.y, a.x);
 }
 
 // a * e^{DIR * 2 pi i m / 16} for a compile-time m
-template <int DIR, int M>
-__device__ __forceinline__ double2 rot16(double2 a)
+template <int DIR, int M, class C>
+__device__ __forceinline__ C rot16(C a)
 {
+    using R = typename CT<C>::R;
     constexpr int m = M & 15;
     constexpr double C1 = 0.92387953251128675613; // cos(pi/8)
     constexpr double S1 = 0.38268343236508977173; // sin(pi/8)
@@ -59,7 +77,7 @@ __device__ __forceinline__ double2 rot16(double2 a)
     } else if constexpr (m == 4) {
         return mul_i<DIR>(a);
     } else if constexpr (m == 8) {
-        return make_double2(-a.x, -a.y);
+        return CT<C>::make(-a.x, -a.y);
     } else if constexpr (m == 12) {
         return mul_i<-DIR>(a);
     } else {
@@ -68,43 +86,42 @@ __device__ __forceinline__ double2 rot16(double2 a)
                                    -1, -C1, -H, -S1, 0, S1, H, C1};
         constexpr double sn[16] = {0, S1, H, C1, 1, C1, H, S1,
                                    0, -S1, -H, -C1, -1, -C1, -H, -S1};
-        constexpr double c = cs[m];
-        constexpr double s = DIR * sn[m];
-        return make_double2(a.x * c - a.y * s, a.x * s + a.y * c);
+        constexpr R c = (R)cs[m];
+        constexpr R s = (R)(DIR * sn[m]);
+        return CT<C>::make(a.x * c - a.y * s, a.x * s + a.y * c);
     }
 }
 
-template <int DIR>
-__device__ __forceinline__ void dft2(double2 *v)
+template <int DIR, class C>
+__device__ __forceinline__ void dft2(C *v)
 {
-    double2 t = v[0];
+    C t = v[0];
     v[0] = cadd(t, v[1]);
     v[1] = csub(t, v[1]);
 }
 
-template <int DIR>
-__device__ __forceinline__ void dft4(double2 &x0, double2 &x1, double2 &x2,
-                                     double2 &x3)
+template <int DIR, class C>
+__device__ __forceinline__ void dft4(C &x0, C &x1, C &x2, C &x3)
 {
-    double2 a0 = cadd(x0, x2), a1 = csub(x0, x2);
-    double2 b0 = cadd(x1, x3), b1 = mul_i<DIR>(csub(x1, x3));
+    C a0 = cadd(x0, x2), a1 = csub(x0, x2);
+    C b0 = cadd(x1, x3), b1 = mul_i<DIR>(csub(x1, x3));
     x0 = cadd(a0, b0);
     x2 = csub(a0, b0);
     x1 = cadd(a1, b1);
     x3 = csub(a1, b1);
 }
 
-template <int DIR>
-__device__ __forceinline__ void dft4(double2 *v)
+template <int DIR, class C>
+__device__ __forceinline__ void dft4(C *v)
 {
     dft4<DIR>(v[0], v[1], v[2], v[3]);
 }
 
-template <int DIR>
-__device__ __forceinline__ void dft8(double2 *v)
+template <int DIR, class C>
+__device__ __forceinline__ void dft8(C *v)
 {
-    double2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
-    double2 o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    C e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+    C o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
     dft4<DIR>(e0, e1, e2, e3);
     dft4<DIR>(o0, o1, o2, o3);
     o1 = rot16<DIR, 2>(o1);
@@ -121,10 +138,10 @@ __device__ __forceinline__ void dft8(double2 *v)
 }
 
 // 16 = 4 x 4: n = 4 p + q, k = k1 + 4 k2
-template <int DIR>
-__device__ __forceinline__ void dft16(double2 *v)
+template <int DIR, class C>
+__device__ __forceinline__ void dft16(C *v)
 {
-    double2 y[4][4];
+    C y[4][4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         y[q][0] = v[q];
@@ -152,8 +169,8 @@ __device__ __forceinline__ void dft16(double2 *v)
     }
 }
 
-template <int R, int DIR>
-__device__ __forceinline__ void dft(double2 *v)
+template <int R, int DIR, class C>
+__device__ __forceinline__ void dft(C *v)
 {
     if constexpr (R == 2)
         dft2<DIR>(v);
@@ -202,16 +219,16 @@ struct Lds {
 // (point n; slot = m * R + r is the register the value lands in) instead of
 // LDS; when LAST_TO_WR, outputs go to wr(slot, n, value) instead of LDS.
 template <int L, int R, int Ns, int DIR, bool FIRST, bool LAST_TO_WR,
-          class P = Plan<L>, class Rd, class Wr>
-__device__ __forceinline__ void stockham_stage(double2 *line, int j,
-                                               const double2 *__restrict__ tw,
+          class P = Plan<L>, class C, class Rd, class Wr>
+__device__ __forceinline__ void stockham_stage(C *line, int j,
+                                               const C *__restrict__ tw,
                                                Rd &rd, Wr &wr)
 {
     constexpr int E = P::E;
     constexpr int T = L / E;
     constexpr int NB = E / R; // butterflies per thread
     constexpr int STRIDE_IN = L / R;
-    double2 v[NB][R];
+    C v[NB][R];
 #pragma unroll
     for (int m = 0; m < NB; ++m) {
         const int jb = j + m * T;
@@ -234,7 +251,7 @@ __device__ __forceinline__ void stockham_stage(double2 *line, int j,
             constexpr int TS = L / (Ns * R);
 #pragma unroll
             for (int r = 1; r < R; ++r) {
-                double2 w = tw[L + k * r * TS];
+                C w = tw[L + k * r * TS];
                 if constexpr (DIR > 0)
                     w = cconj(w);
                 v[m][r] = cmul(v[m][r], w);
@@ -294,7 +311,8 @@ struct NoRead {
     __device__ double2 operator()(int, int) const { return make_double2(0.0, 0.0); }
 };
 struct NoWrite {
-    __device__ void operator()(int, int, double2) const {}
+    template <class C>
+    __device__ void operator()(int, int, C) const {}
 };
 
 // Full line transform.  On entry either the line is in LDS (FIRST_FROM_RD =
@@ -303,9 +321,10 @@ struct NoWrite {
 // unless LAST_TO_WR: then `wr` receives every output point and the LDS line
 // may still be read by other threads of the block (no trailing barrier).
 template <int L, int DIR, bool FIRST_FROM_RD = false, bool LAST_TO_WR = false,
-          class P = Plan<L>, class Rd = NoRead, class Wr = NoWrite>
-__device__ __forceinline__ void fft_line(double2 *line, int j,
-                                         const double2 *__restrict__ tw,
+          class P = Plan<L>, class C = double2, class Rd = NoRead,
+          class Wr = NoWrite>
+__device__ __forceinline__ void fft_line(C *line, int j,
+                                         const C *__restrict__ tw,
                                          Rd rd = Rd(), Wr wr = Wr())
 {
     static_assert(P::E % P::R0 == 0 && P::E % P::R1 == 0 &&
@@ -313,17 +332,17 @@ __device__ __forceinline__ void fft_line(double2 *line, int j,
                   "every radix divides E");
     static_assert(P::R0 * P::R1 * P::R2 * P::R3 == L, "radices multiply to L");
     constexpr int NST = 1 + (P::R1 > 1) + (P::R2 > 1) + (P::R3 > 1);
-    stockham_stage<L, P::R0, 1, DIR, FIRST_FROM_RD, LAST_TO_WR && NST == 1, P>(
-        line, j, tw, rd, wr);
+    stockham_stage<L, P::R0, 1, DIR, FIRST_FROM_RD, LAST_TO_WR && NST == 1, P,
+                   C>(line, j, tw, rd, wr);
     if constexpr (P::R1 > 1)
-        stockham_stage<L, P::R1, P::R0, DIR, false, LAST_TO_WR && NST == 2, P>(
-            line, j, tw, rd, wr);
+        stockham_stage<L, P::R1, P::R0, DIR, false, LAST_TO_WR && NST == 2, P,
+                       C>(line, j, tw, rd, wr);
     if constexpr (P::R2 > 1)
         stockham_stage<L, P::R2, P::R0 * P::R1, DIR, false,
-                       LAST_TO_WR && NST == 3, P>(line, j, tw, rd, wr);
+                       LAST_TO_WR && NST == 3, P, C>(line, j, tw, rd, wr);
     if constexpr (P::R3 > 1)
         stockham_stage<L, P::R3, P::R0 * P::R1 * P::R2, DIR, false, LAST_TO_WR,
-                       P>(line, j, tw, rd, wr);
+                       P, C>(line, j, tw, rd, wr);
 }
 
 } // namespace ihg

@@ -43,13 +43,12 @@ struct LineSet {
 // loads straight from HBM into registers and the last stage stores straight
 // from registers; only the transposes between stages go through LDS.
 // ---------------------------------------------------------------------------
-template <int L, int DIR>
+template <int L, int DIR, class C = double2>
 __global__ __launch_bounds__(FFT_WG) void k_c2c_strided(
-    const double2 *in, double2 *out, LineSet ls, long batch,
-    const double2 *__restrict__ tw)
+    const C *in, C *out, LineSet ls, long batch, const C *__restrict__ tw)
 {
     using TL = Tile<L>;
-    __shared__ double2 lds[TL::B * TL::PITCH];
+    __shared__ C lds[TL::B * TL::PITCH];
     const int tid = threadIdx.x;
     in += blockIdx.y * batch;
     out += blockIdx.y * batch;
@@ -57,14 +56,14 @@ __global__ __launch_bounds__(FFT_WG) void k_c2c_strided(
     const long line = (long)blockIdx.x * TL::B + b;
     const bool act = line < ls.nlines;
     const long base = ls.base(act ? line : 0);
-    const double2 *src = in + base;
-    double2 *dst = out + base;
+    const C *src = in + base;
+    C *dst = out + base;
     const long st = ls.stride;
     auto rd = [&](int, int n) {
-        const double2 v = src[n * st]; // in bounds for every lane
-        return act ? v : make_double2(0.0, 0.0);
+        const C v = src[n * st]; // in bounds for every lane
+        return act ? v : CT<C>::make(0, 0);
     };
-    auto wr = [&](int, int n, double2 v) {
+    auto wr = [&](int, int n, C v) {
         if (act) dst[n * st] = v;
     };
     fft_line<L, DIR, true, true>(lds + b * TL::PITCH, j, tw, rd, wr);
@@ -82,35 +81,36 @@ __global__ __launch_bounds__(FFT_WG) void k_c2c_strided(
 // false the input is already the full spectrum and only the backward
 // transforms run.
 // ---------------------------------------------------------------------------
-template <int L, int NQ, bool FWD, class Fac>
-__global__ __launch_bounds__(FFT_WG) void k_x_fanout(
+template <int L, int NQ, bool FWD, class Fac, class P = Plan<L>,
+          int WG = FFT_WG>
+__global__ __launch_bounds__(WG) void k_x_fanout(
     const double2 *in, double2 *out, long qstride, LineSet ls, long batch,
     const double2 *__restrict__ tw, Fac fac)
 {
-    using TL = Tile<L>;
-    __shared__ double2 lds[TL::B * TL::PITCH];
+    constexpr int E = P::E, T = L / E, B = WG / T, PITCH = Lds<L>::PITCH;
+    __shared__ double2 lds[B * PITCH];
     const int tid = threadIdx.x;
     in += blockIdx.y * batch;
     out += blockIdx.y * batch;
-    const int b = tid % TL::B, j = tid / TL::B;
-    const long line = (long)blockIdx.x * TL::B + b;
+    const int b = tid % B, j = tid / B;
+    const long line = (long)blockIdx.x * B + b;
     const bool act = line < ls.nlines;
     const long base = ls.base(act ? line : 0);
     const double2 *src = in + base;
     const long st = ls.stride;
-    double2 *my = lds + b * TL::PITCH;
-    double2 spec[TL::E];
+    double2 *my = lds + b * PITCH;
+    double2 spec[E];
     auto rd = [&](int, int n) {
         const double2 v = src[n * st]; // in bounds for every lane
         return act ? v : make_double2(0.0, 0.0);
     };
     if constexpr (FWD) {
         auto keep = [&](int slot, int, double2 v) { spec[slot] = v; };
-        fft_line<L, -1, true, true>(my, j, tw, rd, keep);
+        fft_line<L, -1, true, true, P>(my, j, tw, rd, keep);
     } else {
 #pragma unroll
-        for (int slot = 0; slot < TL::E; ++slot)
-            spec[slot] = rd(slot, stage1_point<L>(j, slot));
+        for (int slot = 0; slot < E; ++slot)
+            spec[slot] = rd(slot, stage1_point<L, P>(j, slot));
     }
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
@@ -118,14 +118,20 @@ __global__ __launch_bounds__(FFT_WG) void k_x_fanout(
             __syncthreads(); // LDS reads of the previous transform are done
         double2 *dst = out + q * qstride + base;
         auto rq = [&](int slot, int n) {
-            return fac(q, line, n, spec[FWD ? chain_slot<L>(slot) : slot]);
+            return fac(q, line, n, spec[FWD ? chain_slot<L, P>(slot) : slot]);
         };
         auto wq = [&](int, int n, double2 v) {
             if (act) dst[n * st] = v;
         };
-        fft_line<L, +1, true, true>(my, j, tw, rq, wq);
+        fft_line<L, +1, true, true, P>(my, j, tw, rq, wq);
     }
 }
+
+// Line plans of the derivative fan-out.  (At 256 points, 8 values per
+// thread in three stages -- PlanT<8, 8, 8, 4> on 512 threads, 2 waves per
+// SIMD instead of 1 -- measured 203 us against 198 us for the two-stage
+// 16-value plan, so the default plans stay.)
+template <int L> struct FanPlan { using P = Plan<L>; static constexpr int WG = FFT_WG; };
 
 // ---------------------------------------------------------------------------
 // z passes
@@ -133,15 +139,16 @@ __global__ __launch_bounds__(FFT_WG) void k_x_fanout(
 
 // Hermitian split after a forward transform of Z = A + iB (A, B real rows):
 // A[k] = (Z[k] + conj Z[L-k]) / 2, B[k] = -i (Z[k] - conj Z[L-k]) / 2.
-template <int L>
-__device__ __forceinline__ double2 herm_split(const double2 *line, int k,
-                                              int s)
+template <int L, class C>
+__device__ __forceinline__ C herm_split(const C *line, int k, int s)
 {
-    const double2 z1 = line[Lds<L>::idx(k)];
-    const double2 z2 = cconj(line[Lds<L>::idx((L - k) & (L - 1))]);
+    using R = typename CT<C>::R;
+    const C z1 = line[Lds<L>::idx(k)];
+    const C z2 = cconj(line[Lds<L>::idx((L - k) & (L - 1))]);
+    const R h = (R)0.5;
     if (s == 0)
-        return make_double2(0.5 * (z1.x + z2.x), 0.5 * (z1.y + z2.y));
-    return make_double2(0.5 * (z1.y - z2.y), -0.5 * (z1.x - z2.x));
+        return CT<C>::make(h * (z1.x + z2.x), h * (z1.y + z2.y));
+    return CT<C>::make(h * (z1.y - z2.y), -h * (z1.x - z2.x));
 }
 
 // Merge of two half rows A, B (Hermitian extension, imaginary parts of the

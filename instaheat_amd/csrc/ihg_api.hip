@@ -179,7 +179,7 @@ extern "C" int ihg_destroy(ihg_ctx *c)
     }
     for (hipEvent_t e : c->ev_pool)
         (void)hipEventDestroy(e);
-    void *dev[] = {c->tw, c->filter, c->W, c->Wc, c->grad3, c->rho,
+    void *dev[] = {c->tw, c->twf, c->filter, c->W, c->Wc, c->grad3, c->rho,
                    c->partials, c->dscal, c->dobs};
     for (void *p : dev)
         if (p) (void)hipFree(p);
@@ -253,6 +253,16 @@ extern "C" int ihg_create(const ihg_params *p, ihg_ctx **out)
         if ((rc = dalloc(&c->tw, 1024)) == 0)
             rc = hip_check(hipMemcpy(c->tw, h, 1024 * sizeof(double2),
                                      hipMemcpyHostToDevice), "hipMemcpy");
+        if (!rc) {
+            float2 *hf = (float2 *)calloc(1024, sizeof(float2));
+            for (int i = 0; i < 1024; ++i)
+                hf[i] = make_float2((float)h[i].x, (float)h[i].y);
+            if ((rc = hip_check(hipMalloc(&c->twf, 1024 * sizeof(float2)),
+                                "hipMalloc")) == 0)
+                rc = hip_check(hipMemcpy(c->twf, hf, 1024 * sizeof(float2),
+                                         hipMemcpyHostToDevice), "hipMemcpy");
+            free(hf);
+        }
         free(h);
         if (rc) return fail(rc);
     }
@@ -429,6 +439,13 @@ extern "C" int ihg_mk_rhs(ihg_ctx *c, double t, double *f, double *result,
     return 0;
 }
 
+extern "C" int ihg_set_gw_fp32(ihg_ctx *c, int on)
+{
+    if (!c) return set_error("NULL context");
+    c->p.gw_fp32 = on ? 1 : 0;
+    return 0;
+}
+
 extern "C" int ihg_features(ihg_ctx *c, const double *f, double out[3])
 {
     if (!c || !out) return set_error("NULL argument");
@@ -487,6 +504,20 @@ extern "C" int ihg_aux(ihg_ctx *c, int which, double *out)
     const double *src = nullptr;
     if (which >= 0 && which <= 2) {
         if (!c->grad3) return set_error("gradients are kept only with GW on");
+        if (c->p.gw_fp32) {
+            // the precision form keeps them in FP32
+            float *tmp = (float *)malloc(c->N * sizeof(float));
+            if (!tmp) return set_error("out of host memory");
+            int rc = hip_check(hipMemcpyAsync(tmp,
+                                              reinterpret_cast<const float *>(c->grad3) + which * c->N,
+                                              c->N * sizeof(float),
+                                              hipMemcpyDeviceToHost, c->stream),
+                               "download gradients");
+            if (!rc) rc = hip_check(hipStreamSynchronize(c->stream), "sync");
+            for (long i = 0; !rc && i < c->N; ++i) out[i] = tmp[i];
+            free(tmp);
+            return rc;
+        }
         src = c->grad3 + which * c->N;
     } else if (which == 3) {
         src = c->rho;

@@ -21,21 +21,21 @@ static LineSet x_lines(const ihg_ctx *c)
     return LineSet{plane, plane, 0, plane};
 }
 
-template <int L>
-static int launch_c2c(ihg_ctx *c, const double2 *in, double2 *out,
-                      LineSet ls, int dir, int batch, long cbatch, int fam)
+template <int L, class C>
+static int launch_c2c(ihg_ctx *c, const C *in, C *out, LineSet ls, int dir,
+                      int batch, long cbatch, int fam, const C *tw)
 {
     using TL = Tile<L>;
     dim3 grid((unsigned)((ls.nlines + TL::B - 1) / TL::B), batch);
     // algorithmic bytes: every complex value read once and written once
-    const double bytes = 2.0 * 16.0 * c->M * batch;
+    const double bytes = 2.0 * sizeof(C) * c->M * batch;
     return timed(c, fam, bytes, [&]() {
         if (dir < 0)
-            hipLaunchKernelGGL((k_c2c_strided<L, -1>), grid, dim3(FFT_WG), 0,
-                               c->stream, in, out, ls, cbatch, c->tw);
+            hipLaunchKernelGGL((k_c2c_strided<L, -1, C>), grid, dim3(FFT_WG), 0,
+                               c->stream, in, out, ls, cbatch, tw);
         else
-            hipLaunchKernelGGL((k_c2c_strided<L, +1>), grid, dim3(FFT_WG), 0,
-                               c->stream, in, out, ls, cbatch, c->tw);
+            hipLaunchKernelGGL((k_c2c_strided<L, +1, C>), grid, dim3(FFT_WG), 0,
+                               c->stream, in, out, ls, cbatch, tw);
         return hip_check(hipGetLastError(), "k_c2c_strided");
     });
 }
@@ -44,7 +44,7 @@ int y_pass(ihg_ctx *c, double2 *data, int dir, int batch, long cbatch)
 {
     return dispatch_len(c->Ny, [&](auto Lc) {
         return launch_c2c<decltype(Lc)::value>(c, data, data, y_lines(c), dir,
-                                               batch, cbatch, F_Y_PASS);
+                                               batch, cbatch, F_Y_PASS, c->tw);
     });
 }
 
@@ -52,7 +52,15 @@ int x_pass(ihg_ctx *c, double2 *data, int dir, int batch, long cbatch)
 {
     return dispatch_len(c->Nx, [&](auto Lc) {
         return launch_c2c<decltype(Lc)::value>(c, data, data, x_lines(c), dir,
-                                               batch, cbatch, F_X_PASS);
+                                               batch, cbatch, F_X_PASS, c->tw);
+    });
+}
+
+int x_pass_f32(ihg_ctx *c, float2 *data, int dir, int batch, long cbatch)
+{
+    return dispatch_len(c->Nx, [&](auto Lc) {
+        return launch_c2c<decltype(Lc)::value>(c, data, data, x_lines(c), dir,
+                                               batch, cbatch, F_X_PASS, c->twf);
     });
 }
 
@@ -61,7 +69,7 @@ static int x_pass_oop(ihg_ctx *c, const double2 *in, double2 *out, int dir,
 {
     return dispatch_len(c->Nx, [&](auto Lc) {
         return launch_c2c<decltype(Lc)::value>(c, in, out, x_lines(c), dir,
-                                               batch, cbatch, F_X_PASS);
+                                               batch, cbatch, F_X_PASS, c->tw);
     });
 }
 

@@ -78,6 +78,7 @@ struct ihg_ctx {
     ihg::KGeom kg;
     hipStream_t stream;
     double2 *tw;       // twiddle tables, tw[L + m] = e^{-2 pi i m / L}
+    float2 *twf;       // the same rounded to FP32 (FP32 GW source path)
     double *filter;    // filter mask, M values (filter in the reference)
     double2 *W;        // 6 spectral work arrays (M complex each)
     double2 *Wc;       // 2 more spectral arrays (phic / psic for output)
@@ -113,6 +114,7 @@ int fft_c2r(ihg_ctx *c, const double2 *in, double *out, double2 *work,
             int batch, long cbatch, long rbatch);
 int y_pass(ihg_ctx *c, double2 *data, int dir, int batch, long cbatch);
 int x_pass(ihg_ctx *c, double2 *data, int dir, int batch, long cbatch);
+int x_pass_f32(ihg_ctx *c, float2 *data, int dir, int batch, long cbatch);
 int z_r2c_pass(ihg_ctx *c, const double *in, double2 *out, int batch,
                long rbatch, long cbatch);
 int z_c2r_pass(ihg_ctx *c, const double2 *in, double *out, int batch,

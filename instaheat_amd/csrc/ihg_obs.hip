@@ -234,49 +234,55 @@ __global__ __launch_bounds__(RED_WG) void k_power_spectrum(
         o[b] = hist[b];
 }
 
-// mk_gw_spectrum (src/toolbox.c:435-484)
+// mk_gw_spectrum (src/toolbox.c:435-484).  T = double: the reference's
+// arithmetic; T = float: the precision form (per-mode power and per-block
+// bin sums in FP32; bins from the FP64 |k|, cross-block sums in FP64)
+template <class T>
 __global__ __launch_bounds__(RED_WG) void k_gw_spectrum(
     const double *__restrict__ f, long N, long Next, long M, KGeom kg, int bins,
     double lxyz, const double *__restrict__ dscal, double *__restrict__ part)
 {
-    __shared__ double hist[MAX_BINS];
+    __shared__ T hist[MAX_BINS];
     for (int b = threadIdx.x; b < bins; b += RED_WG)
-        hist[b] = 0.0;
+        hist[b] = 0;
     __syncthreads();
     const long Ndh1 = 4 * N + 2 * Next, Ndh2 = Ndh1 + Next;
     const double hubble = sqrt(dscal[S_RHO_MEAN] / 3.0);
     const double rat = pow(0.01, 1.0 / 3.0);
-    const double fac = M_PI * rat / (3.0 * hubble * hubble * lxyz);
+    const T fac = (T)(M_PI * rat / (3.0 * hubble * hubble * lxyz));
+    const T two = 2;
     const long stride = (long)gridDim.x * RED_WG;
     for (long i = (long)blockIdx.x * RED_WG + threadIdx.x; i < M; i += stride) {
         if (i == 0) continue;
         int ix, iy, iz;
         mode_index(kg, i, ix, iy, iz);
-        const double kx = kg.kxf(ix), ky = kg.kyf(iy), kz = kg.kzf(iz);
-        const double kx2 = kx * kx, ky2 = ky * ky, kz2 = kz * kz;
-        const double k2 = kg.ksq(ix, iy, iz), k = sqrt(k2);
-        const double dh1r = f[Ndh1 + 2 * i], dh2r = f[Ndh2 + 2 * i];
-        const double dh1i = f[Ndh1 + 2 * i + 1], dh2i = f[Ndh2 + 2 * i + 1];
-        double pw;
-        if (fabs(kz) > DBL_EPSILON) {
-            pw = 2.0 * k2 / (kz2 * (ky2 + kz2)) *
+        const double kxd = kg.kxf(ix), kyd = kg.kyf(iy), kzd = kg.kzf(iz);
+        const double k2d = kg.ksq(ix, iy, iz), kd = sqrt(k2d);
+        const T kx = (T)kxd, ky = (T)kyd, kz = (T)kzd;
+        const T kx2 = kx * kx, ky2 = ky * ky, kz2 = kz * kz;
+        const T k2 = (T)k2d, k = (T)kd;
+        const T dh1r = (T)f[Ndh1 + 2 * i], dh2r = (T)f[Ndh2 + 2 * i];
+        const T dh1i = (T)f[Ndh1 + 2 * i + 1], dh2i = (T)f[Ndh2 + 2 * i + 1];
+        T pw;
+        if (fabs(kzd) > DBL_EPSILON) {
+            pw = two * k2 / (kz2 * (ky2 + kz2)) *
                  ((kx2 + kz2) * (dh1r * dh1r + dh1i * dh1i) +
-                  2.0 * kx * ky * (dh1r * dh2r + dh1i * dh2i) +
+                  two * kx * ky * (dh1r * dh2r + dh1i * dh2i) +
                   (ky2 + kz2) * (dh2r * dh2r + dh2i * dh2i));
-        } else if (fabs(ky) > DBL_EPSILON) {
-            pw = 2.0 * (kx2 + ky2) / (ky2 * ky2) *
+        } else if (fabs(kyd) > DBL_EPSILON) {
+            pw = two * (kx2 + ky2) / (ky2 * ky2) *
                  ((kx2 + ky2) * (dh1r * dh1r + dh1i * dh1i) +
                   ky2 * (dh2r * dh2r + dh2i * dh2i));
         } else {
-            pw = 2.0 * (dh1r * dh1r + dh1i * dh1i + dh2r * dh2r + dh2i * dh2i);
+            pw = two * (dh1r * dh1r + dh1i * dh1i + dh2r * dh2r + dh2i * dh2i);
         }
-        if (fabs(kg.kzd(iz)) > DBL_EPSILON) pw *= 2.0;
-        atomicAdd(&hist[bin_of(k, kg.k_max, bins)], fac * k2 * k * pw);
+        if (fabs(kg.kzd(iz)) > DBL_EPSILON) pw *= two;
+        atomicAdd(&hist[bin_of(kd, kg.k_max, bins)], fac * k2 * k * pw);
     }
     __syncthreads();
     double *o = part + (long)blockIdx.x * bins;
     for (int b = threadIdx.x; b < bins; b += RED_WG)
-        o[b] = hist[b];
+        o[b] = (double)hist[b];
 }
 
 // sum of the per-block histograms in block order; out = sum / div
@@ -375,9 +381,14 @@ int rhs_output(ihg_ctx *c, const double *f, int kept)
         const unsigned nbm = red_blocks(M);
         const double lxyz = (c->p.xb - c->p.xa) * (c->p.yb - c->p.ya) *
                             (c->p.zb - c->p.za);
-        hipLaunchKernelGGL(k_gw_spectrum, dim3(nbm), dim3(RED_WG), 0,
-                           c->stream, f, N, c->Next, M, c->kg, bins, lxyz,
-                           c->dscal, c->partials);
+        if (c->p.gw_fp32)
+            hipLaunchKernelGGL(k_gw_spectrum<float>, dim3(nbm), dim3(RED_WG), 0,
+                               c->stream, f, N, c->Next, M, c->kg, bins, lxyz,
+                               c->dscal, c->partials);
+        else
+            hipLaunchKernelGGL(k_gw_spectrum<double>, dim3(nbm), dim3(RED_WG),
+                               0, c->stream, f, N, c->Next, M, c->kg, bins,
+                               lxyz, c->dscal, c->partials);
         IHG_LAUNCHED("k_gw_spectrum");
         hipLaunchKernelGGL(k_hist_final, dim3((bins + RED_WG - 1) / RED_WG, 1),
                            dim3(RED_WG), 0, c->stream, c->partials, (int)nbm,

@@ -17,6 +17,7 @@
 // update (toolbox.c:311-424).
 #include <cfloat>
 #include <cmath>
+#include <type_traits>
 
 #include "dispatch.h"
 #include "fft_kernels.h"
@@ -84,10 +85,10 @@ struct FilterFac {
 // and mk_rho_and_p (src/toolbox.c:196-243).  Line 2r holds (d_x, d_y) of row
 // r, line 2r+1 holds (d_z, lap).
 // ---------------------------------------------------------------------------
-template <int L>
+template <int L, class G>
 __global__ __launch_bounds__(FFT_WG) void k_z_c2r_rho(
     const double2 *__restrict__ W, long M, const double *__restrict__ f,
-    double *__restrict__ result, double *__restrict__ grad3,
+    double *__restrict__ result, G *__restrict__ grad3,
     double *__restrict__ rho, double *__restrict__ pressure,
     double *__restrict__ partials, long nrows, long N, double m2,
     const double2 *__restrict__ tw)
@@ -135,9 +136,9 @@ __global__ __launch_bounds__(FFT_WG) void k_z_c2r_rho(
         result[N + i] = lap;
         result[3 * N + i] = pv;
         if (grad3) {
-            grad3[i] = dx;
-            grad3[N + i] = dy;
-            grad3[2 * N + i] = dz;
+            grad3[i] = (G)dx;
+            grad3[N + i] = (G)dy;
+            grad3[2 * N + i] = (G)dz;
         }
         if (rho) {
             rho[i] = rv;
@@ -208,18 +209,18 @@ __global__ __launch_bounds__(RED_WG) void k_update_phi_psi(
 // the three complex rows (S11, S12), (S13, S22), (S23, S33) of each row are
 // transformed one after the other; component c = 2q + s goes to W + c M.
 // ---------------------------------------------------------------------------
-template <int L, int Q, class Rows>
-__device__ __forceinline__ void sources_pair(double2 *lds, int tid,
-                                             const Rows &g, bool act,
-                                             double2 *__restrict__ W, long M,
-                                             long row0, long nrows,
-                                             const double2 *__restrict__ tw)
+template <int L, int Q, class C, class Rows>
+__device__ __forceinline__ void sources_pair(C *lds, int tid, const Rows &g,
+                                             bool act, C *__restrict__ W,
+                                             long M, long row0, long nrows,
+                                             const C *__restrict__ tw)
 {
     using TL = Tile<L>;
+    using R = typename CT<C>::R;
     constexpr int MZ = L / 2 + 1;
     const int bc = tid / TL::T;
     auto rd = [&](int s, int) {
-        double u, v;
+        R u, v;
         if constexpr (Q == 0) {
             u = g.x[s] * g.x[s];
             v = g.x[s] * g.y[s];
@@ -230,7 +231,7 @@ __device__ __forceinline__ void sources_pair(double2 *lds, int tid,
             u = g.y[s] * g.z[s];
             v = g.z[s] * g.z[s];
         }
-        return act ? make_double2(u, v) : make_double2(0.0, 0.0);
+        return act ? CT<C>::make(u, v) : CT<C>::make(0, 0);
     };
     fft_line<L, -1, true>(lds + bc * TL::PITCH, tid % TL::T, tw, rd);
     for (int idx = tid; idx < TL::B * 2 * MZ; idx += FFT_WG) {
@@ -245,25 +246,28 @@ __device__ __forceinline__ void sources_pair(double2 *lds, int tid,
     __syncthreads(); // split reads done before the next transform
 }
 
-template <int L>
+template <int L, class R>
 struct GradRegs {
-    double x[Plan<L>::E], y[Plan<L>::E], z[Plan<L>::E];
+    R x[Plan<L>::E], y[Plan<L>::E], z[Plan<L>::E];
 };
 
-template <int L>
+// C = double2: the FP64 path; C = float2: the precision form (gradients
+// stored in FP32, products, FFTs and output in FP32)
+template <int L, class C>
 __global__ __launch_bounds__(FFT_WG) void k_z_r2c_sources(
-    const double *__restrict__ g3, double2 *__restrict__ W, long M,
-    long nrows, long N, const double2 *__restrict__ tw)
+    const typename CT<C>::R *__restrict__ g3, C *__restrict__ W, long M,
+    long nrows, long N, const C *__restrict__ tw)
 {
     using TL = Tile<L>;
-    __shared__ double2 lds[TL::B * TL::PITCH];
+    using R = typename CT<C>::R;
+    __shared__ C lds[TL::B * TL::PITCH];
     const int tid = threadIdx.x;
     const int bc = tid / TL::T, j = tid % TL::T;
     const long row0 = (long)blockIdx.x * TL::B;
     const long row = row0 + bc;
     const bool act = row < nrows;
-    const double *gx = g3 + (act ? row : 0) * L;
-    GradRegs<L> g;
+    const R *gx = g3 + (act ? row : 0) * L;
+    GradRegs<L, R> g;
 #pragma unroll
     for (int s = 0; s < TL::E; ++s) {
         const int n = stage1_point<L>(j, s);
@@ -287,78 +291,79 @@ __global__ __launch_bounds__(FFT_WG) void k_z_r2c_sources(
 // alpha_C, beta_C are the derivatives of these with respect to component
 // C = 0..5 = (xx, xy, xz, yy, yz, zz); the k = 0 mode projects to 0.
 // ---------------------------------------------------------------------------
-template <int C>
-__device__ __forceinline__ void tt_coef(double kx, double ky, double kz,
-                                        double inv_ksq, bool zero, double &al,
-                                        double &be)
+template <int C, class R>
+__device__ __forceinline__ void tt_coef(R kx, R ky, R kz, R inv_ksq,
+                                        bool b1, bool zero, R &al, R &be)
 {
-    const double fx = kx * inv_ksq, fy = ky * inv_ksq, fz = kz * inv_ksq;
-    const bool b1 = fabs(kz) > DBL_EPSILON;
-    const bool b2 = !b1 && fabs(ky) > DBL_EPSILON;
+    const R fx = kx * inv_ksq, fy = ky * inv_ksq, fz = kz * inv_ksq;
+    const bool b2 = !b1 && fabs((double)ky) > DBL_EPSILON;
     const bool py = !b1 && !b2; // p = y, else p = x
     const bool qy = b1;         // q = y, else q = z
-    const double kp = py ? ky : kx, fp = py ? fy : fx;
-    const double kq = qy ? ky : kz, fq = qy ? fy : fz;
-    double t1, t2, rowp, rowq, dpp, dpq;
+    const R kp = py ? ky : kx, fp = py ? fy : fx;
+    const R kq = qy ? ky : kz, fq = qy ? fy : fz;
+    R t1, t2, rowp, rowq, dpp, dpq;
+    const R O = 0, I = 1, D = 2;
     if constexpr (C == 0) {
-        t1 = kx * fx; t2 = 1.0; rowp = py ? 0.0 : kx; rowq = 0.0;
-        dpp = py ? 0.0 : 1.0; dpq = 0.0;
+        t1 = kx * fx; t2 = I; rowp = py ? O : kx; rowq = O;
+        dpp = py ? O : I; dpq = O;
     } else if constexpr (C == 1) {
-        t1 = 2.0 * (kx * fy); t2 = 0.0; rowp = py ? kx : ky;
-        rowq = qy ? kx : 0.0; dpp = 0.0; dpq = (!py && qy) ? 1.0 : 0.0;
+        t1 = D * (kx * fy); t2 = O; rowp = py ? kx : ky;
+        rowq = qy ? kx : O; dpp = O; dpq = (!py && qy) ? I : O;
     } else if constexpr (C == 2) {
-        t1 = 2.0 * (kx * fz); t2 = 0.0; rowp = py ? 0.0 : kz;
-        rowq = qy ? 0.0 : kx; dpp = 0.0; dpq = (!py && !qy) ? 1.0 : 0.0;
+        t1 = D * (kx * fz); t2 = O; rowp = py ? O : kz;
+        rowq = qy ? O : kx; dpp = O; dpq = (!py && !qy) ? I : O;
     } else if constexpr (C == 3) {
-        t1 = ky * fy; t2 = 1.0; rowp = py ? ky : 0.0; rowq = qy ? ky : 0.0;
-        dpp = py ? 1.0 : 0.0; dpq = 0.0;
+        t1 = ky * fy; t2 = I; rowp = py ? ky : O; rowq = qy ? ky : O;
+        dpp = py ? I : O; dpq = O;
     } else if constexpr (C == 4) {
-        t1 = 2.0 * (ky * fz); t2 = 0.0; rowp = py ? kz : 0.0;
-        rowq = qy ? kz : ky; dpp = 0.0; dpq = py ? 1.0 : 0.0;
+        t1 = D * (ky * fz); t2 = O; rowp = py ? kz : O;
+        rowq = qy ? kz : ky; dpp = O; dpq = py ? I : O;
     } else {
-        t1 = kz * fz; t2 = 1.0; rowp = 0.0; rowq = qy ? 0.0 : kz;
-        dpp = 0.0; dpq = 0.0;
+        t1 = kz * fz; t2 = I; rowp = O; rowq = qy ? O : kz;
+        dpp = O; dpq = O;
     }
-    const double s1 = t1 + t2, s2 = t1 - t2;
-    al = zero ? 0.0 : dpp - 2.0 * fp * rowp + 0.5 * (fp * kp * s1 + s2);
-    be = zero ? 0.0 : dpq - fp * rowq - fq * rowp + 0.5 * fp * kq * s1;
+    const R s1 = t1 + t2, s2 = t1 - t2;
+    const R half = (R)0.5, two = (R)2.0;
+    al = zero ? (R)0 : dpp - two * fp * rowp + half * (fp * kp * s1 + s2);
+    be = zero ? (R)0 : dpq - fp * rowq - fq * rowp + half * fp * kq * s1;
 }
 
 // y transform of source component C (its z and x passes are done) with the
 // TT accumulation folded into the last Stockham stage; `cur` holds the
 // component's first-stage points (loaded one component ahead)
-template <int L, class P, int C>
+template <int L, class P, int K, class C>
 __device__ __forceinline__ void tt_component(
-    double2 *my, int j, const double2 *__restrict__ tw, const double2 *cur,
-    const KGeom &kg, int ix, int iz, const double *inv_ksq, double2 *o0,
-    double2 *o1)
+    C *my, int j, const C *__restrict__ tw, const C *cur, const KGeom &kg,
+    int ix, int iz, const typename CT<C>::R *inv_ksq, C *o0, C *o1)
 {
-    const double kx = kg.kxf(ix), kz = kg.kzf(iz);
-    if constexpr (C > 0)
+    using R = typename CT<C>::R;
+    const R kx = (R)kg.kxf(ix), kz = (R)kg.kzf(iz);
+    const bool b1 = fabs(kg.kzf(iz)) > DBL_EPSILON; // branch on the FP64 k
+    if constexpr (K > 0)
         __syncthreads(); // LDS reads of the previous component are done
     // Register pressure control (empty asm statements, no instructions):
     // an opaque copy of the twiddle pointer per component stops the compiler
     // from loading every stage twiddle once and holding it across all six
     // transforms, and pinning the accumulators after each component stops it
     // from deferring the sums (and so keeping all six spectra live).
-    const double2 *twc = tw;
+    const C *twc = tw;
     asm volatile("" : "+s"(twc));
     auto rd = [&](int s, int) { return cur[s]; };
-    auto acc = [&](int s, int n, double2 v) {
+    auto acc = [&](int s, int n, C v) {
         // recomputed per component from 1/k^2 alone: an empty asm keeps the
         // compiler from holding every derived coefficient of every point
         // live across the six transforms
-        double iv = inv_ksq[s];
+        R iv = inv_ksq[s];
         asm volatile("" : "+v"(iv));
         const bool zero = (ix == 0) && (iz == 0) && (n == 0);
-        double al, be;
-        tt_coef<C>(kx, kg.kyf(n), kz, iv, zero, al, be);
-        if constexpr (C == 0) {
-            o0[s] = make_double2(al * v.x, al * v.y);
-            o1[s] = make_double2(be * v.x, be * v.y);
+        R al, be;
+        tt_coef<K>(kx, (R)kg.kyf(n), kz, iv, b1, zero, al, be);
+        if constexpr (K == 0) {
+            o0[s] = CT<C>::make(al * v.x, al * v.y);
+            o1[s] = CT<C>::make(be * v.x, be * v.y);
         } else {
-            o0[s] = make_double2(o0[s].x + al * v.x, o0[s].y + al * v.y);
-            o1[s] = make_double2(o1[s].x + be * v.x, o1[s].y + be * v.y);
+            o0[s] = CT<C>::make(o0[s].x + al * v.x, o0[s].y + al * v.y);
+            o1[s] = CT<C>::make(o1[s].x + be * v.x, o1[s].y + be * v.y);
         }
     };
     fft_line<L, -1, true, true, P>(my, j, twc, rd, acc);
@@ -384,15 +389,15 @@ template <> struct TTPlan<512> { using P = PlanT<8, 8, 4, 4, 4>; static constexp
 // Last pass of the GW sources: y transforms of the six components (z and x
 // passes done), the TT projection, and update_h (src/toolbox.c:311-346):
 //   dh = hdot,  dhdot = -(2 p_mean + k^2/a^2) h - 3 H hdot + 2 S^TT / a^2
-template <int L, int WG, class P>
+template <int L, int WG, class P, class C>
 __global__ __launch_bounds__(WG) void k_y_tt_update_h(
-    const double2 *W, long M, const double *__restrict__ f,
+    const C *W, long M, const double *__restrict__ f,
     double *__restrict__ result, long N, long Next, long Ntot, KGeom kg,
-    const double *__restrict__ dscal, LineSet ls,
-    const double2 *__restrict__ tw)
+    const double *__restrict__ dscal, LineSet ls, const C *__restrict__ tw)
 {
+    using R = typename CT<C>::R;
     constexpr int E = P::E, T = L / E, B = WG / T, PITCH = Lds<L>::PITCH;
-    __shared__ double2 lds[B * PITCH];
+    __shared__ C lds[B * PITCH];
     const int tid = threadIdx.x, b = tid % B, j = tid / B;
     const long line = (long)blockIdx.x * B + b;
     const bool act = line < ls.nlines;
@@ -400,26 +405,26 @@ __global__ __launch_bounds__(WG) void k_y_tt_update_h(
     const long base = ls.base(lc);
     const int ix = (int)(lc / kg.Mz), iz = (int)(lc - (long)ix * kg.Mz);
     const long st = ls.stride;
-    double2 *my = lds + b * PITCH;
-    double inv[E];
+    C *my = lds + b * PITCH;
+    R inv[E];
 #pragma unroll
     for (int s = 0; s < E; ++s) {
         const int n = last_point<L, P>(j, s);
         const bool zero = (ix == 0) && (iz == 0) && (n == 0);
-        inv[s] = zero ? 0.0 : 1.0 / kg.ksq(ix, n, iz);
+        inv[s] = zero ? (R)0 : (R)(1.0 / kg.ksq(ix, n, iz));
     }
     // first-stage points of component c, in bounds for every lane; the
     // loads of component c + 1 are issued before the transform of c (the
     // barriers wait on LDS only), so their latency hides behind it
-    auto load = [&](int c, double2 *buf) {
-        const double2 *src = W + c * M + base;
+    auto load = [&](int c, C *buf) {
+        const C *src = W + c * M + base;
 #pragma unroll
         for (int s = 0; s < E; ++s) {
-            const double2 v = src[(long)stage1_point<L, P>(j, s) * st];
-            buf[s] = act ? v : make_double2(0.0, 0.0);
+            const C v = src[(long)stage1_point<L, P>(j, s) * st];
+            buf[s] = act ? v : CT<C>::make(0, 0);
         }
     };
-    double2 o0[E], o1[E], ba[E], bb[E];
+    C o0[E], o1[E], ba[E], bb[E];
     load(0, ba);
     load(1, bb);
     tt_component<L, P, 0>(my, j, tw, ba, kg, ix, iz, inv, o0, o1);
@@ -453,12 +458,11 @@ __global__ __launch_bounds__(WG) void k_y_tt_update_h(
         r2[Nh1 / 2 + i] = d1;
         r2[Nh2 / 2 + i] = d2;
         const double tmp = -(2.0 * pm + ksq / a2);
-        r2[Ndh1 / 2 + i] =
-            make_double2(tmp * h1.x - h3 * d1.x + 2.0 * o0[s].x / a2,
-                         tmp * h1.y - h3 * d1.y + 2.0 * o0[s].y / a2);
-        r2[Ndh2 / 2 + i] =
-            make_double2(tmp * h2.x - h3 * d2.x + 2.0 * o1[s].x / a2,
-                         tmp * h2.y - h3 * d2.y + 2.0 * o1[s].y / a2);
+        const double s0x = o0[s].x, s0y = o0[s].y, s1x = o1[s].x, s1y = o1[s].y;
+        r2[Ndh1 / 2 + i] = make_double2(tmp * h1.x - h3 * d1.x + 2.0 * s0x / a2,
+                                        tmp * h1.y - h3 * d1.y + 2.0 * s0y / a2);
+        r2[Ndh2 / 2 + i] = make_double2(tmp * h2.x - h3 * d2.x + 2.0 * s1x / a2,
+                                        tmp * h2.y - h3 * d2.y + 2.0 * s1y / a2);
     }
 }
 
@@ -487,19 +491,21 @@ int rhs_core(ihg_ctx *c, double *f, double *result, int output, int kept)
     // x fan-out into the four derivative spectra W0..W3
     IHG_TRY(dispatch_len(c->Nx, [&](auto Lc) {
         constexpr int L = decltype(Lc)::value;
-        using TL = Tile<L>;
+        using FP = FanPlan<L>;
+        using P = typename FP::P;
+        constexpr int B = FP::WG / (L / P::E);
         const LineSet ls = x_lineset(c);
         DerivFac fac{c->kg, Nd};
-        const dim3 grid((unsigned)((ls.nlines + TL::B - 1) / TL::B));
+        const dim3 grid((unsigned)((ls.nlines + B - 1) / B));
         return timed(c, F_X_FANOUT, 5.0 * 16.0 * M, [&]() {
             if (kept)
-                hipLaunchKernelGGL((k_x_fanout<L, 4, false, DerivFac>), grid,
-                                   dim3(FFT_WG), 0, c->stream, W + 4 * M, W, M,
-                                   ls, 0L, c->tw, fac);
+                hipLaunchKernelGGL((k_x_fanout<L, 4, false, DerivFac, P, FP::WG>),
+                                   grid, dim3(FP::WG), 0, c->stream, W + 4 * M,
+                                   W, M, ls, 0L, c->tw, fac);
             else
-                hipLaunchKernelGGL((k_x_fanout<L, 4, true, DerivFac>), grid,
-                                   dim3(FFT_WG), 0, c->stream, W + 4 * M, W, M,
-                                   ls, 0L, c->tw, fac);
+                hipLaunchKernelGGL((k_x_fanout<L, 4, true, DerivFac, P, FP::WG>),
+                                   grid, dim3(FP::WG), 0, c->stream, W + 4 * M,
+                                   W, M, ls, 0L, c->tw, fac);
             return hip_check(hipGetLastError(), "k_x_fanout");
         });
     }));
@@ -515,15 +521,24 @@ int rhs_core(ihg_ctx *c, double *f, double *result, int output, int kept)
         if (2L * nb > c->npartials)
             return set_error("partials buffer too small");
         // 4 spectra in; phi, dphi, psi in; lap, p out (+ gradients, rho, p)
-        const double bytes = 4.0 * 16.0 * M +
-                             8.0 * N * (5 + (gw ? 3 : 0) + (output ? 2 : 0));
+        const double gbytes = (gw && c->p.gw_fp32) ? 4.0 : 8.0;
+        const double bytes = 4.0 * 16.0 * M + 8.0 * N * (5 + (output ? 2 : 0)) +
+                             (gw ? 3.0 * gbytes * N : 0.0);
         return timed(c, F_Z_C2R_RHO, bytes, [&]() {
-            hipLaunchKernelGGL((k_z_c2r_rho<L>), dim3(nb), dim3(FFT_WG), 0,
-                               c->stream, W, M, f, result,
-                               gw ? c->grad3 : nullptr,
-                               output ? c->rho : nullptr,
-                               output ? c->pressure : nullptr, c->partials,
-                               nrows, N, m2, c->tw);
+            if (gw && c->p.gw_fp32)
+                hipLaunchKernelGGL((k_z_c2r_rho<L, float>), dim3(nb),
+                                   dim3(FFT_WG), 0, c->stream, W, M, f, result,
+                                   reinterpret_cast<float *>(c->grad3),
+                                   output ? c->rho : nullptr,
+                                   output ? c->pressure : nullptr, c->partials,
+                                   nrows, N, m2, c->tw);
+            else
+                hipLaunchKernelGGL((k_z_c2r_rho<L, double>), dim3(nb),
+                                   dim3(FFT_WG), 0, c->stream, W, M, f, result,
+                                   gw ? c->grad3 : nullptr,
+                                   output ? c->rho : nullptr,
+                                   output ? c->pressure : nullptr, c->partials,
+                                   nrows, N, m2, c->tw);
             return hip_check(hipGetLastError(), "k_z_c2r_rho");
         });
     }));
@@ -541,22 +556,29 @@ int rhs_core(ihg_ctx *c, double *f, double *result, int output, int kept)
     });
 }
 
-int rhs_gw(ihg_ctx *c, const double *f, double *result)
+// GW source pipeline on complex type C: z r2c of the six products, x pass,
+// fused y pass + TT + update_h.  Wk holds 6 M values of C, g3 the gradients.
+template <class C>
+static int rhs_gw_t(ihg_ctx *c, const typename CT<C>::R *g3, C *Wk,
+                    const C *tw, const double *f, double *result)
 {
     const long N = c->N, M = c->M;
     const long nrows = (long)c->Nx * c->Ny;
-    double2 *W = c->W;
+    const double cs = sizeof(C), rs = sizeof(typename CT<C>::R);
     IHG_TRY(dispatch_len(c->Nz, [&](auto Lc) {
         constexpr int L = decltype(Lc)::value;
         using TL = Tile<L>;
         const unsigned nb = (unsigned)((nrows + TL::B - 1) / TL::B);
-        return timed(c, F_Z_R2C_SRC, 3.0 * 8.0 * N + 6.0 * 16.0 * M, [&]() {
-            hipLaunchKernelGGL((k_z_r2c_sources<L>), dim3(nb), dim3(FFT_WG),
-                               0, c->stream, c->grad3, W, M, nrows, N, c->tw);
+        return timed(c, F_Z_R2C_SRC, 3.0 * rs * N + 6.0 * cs * M, [&]() {
+            hipLaunchKernelGGL((k_z_r2c_sources<L, C>), dim3(nb), dim3(FFT_WG),
+                               0, c->stream, g3, Wk, M, nrows, N, tw);
             return hip_check(hipGetLastError(), "k_z_r2c_sources");
         });
     }));
-    IHG_TRY(x_pass(c, W, -1, 6, M));
+    if constexpr (std::is_same<C, float2>::value)
+        IHG_TRY(x_pass_f32(c, Wk, -1, 6, M));
+    else
+        IHG_TRY(x_pass(c, Wk, -1, 6, M));
     // y pass of the six sources, TT projection and update_h: six sources
     // and h1, h2, dh1, dh2 in; the four h blocks of result out
     return dispatch_len(c->Ny, [&](auto Lc) {
@@ -566,14 +588,25 @@ int rhs_gw(ihg_ctx *c, const double *f, double *result)
         // lines along y: one per (x, kz); points Mz apart
         const LineSet ls{(long)c->Nx * c->Mz, c->Mz, (long)c->Ny * c->Mz, c->Mz};
         const unsigned nb = (unsigned)((ls.nlines + B - 1) / B);
-        return timed(c, F_TT_UPDATE_H, 14.0 * 16.0 * M, [&]() {
-            hipLaunchKernelGGL((k_y_tt_update_h<L, TP::WG, typename TP::P>),
-                               dim3(nb), dim3(TP::WG), 0, c->stream, W, M, f,
+        return timed(c, F_TT_UPDATE_H, (6.0 * cs + 8.0 * 16.0) * M, [&]() {
+            hipLaunchKernelGGL((k_y_tt_update_h<L, TP::WG, typename TP::P, C>),
+                               dim3(nb), dim3(TP::WG), 0, c->stream, Wk, M, f,
                                result, N, c->Next, c->Ntot, c->kg, c->dscal,
-                               ls, c->tw);
+                               ls, tw);
             return hip_check(hipGetLastError(), "k_y_tt_update_h");
         });
     });
+}
+
+int rhs_gw(ihg_ctx *c, const double *f, double *result)
+{
+    if (c->p.gw_fp32)
+        // FP32 values in the FP64 buffers: 6 M float2 fit in W, 3 N floats
+        // in grad3
+        return rhs_gw_t<float2>(c, reinterpret_cast<const float *>(c->grad3),
+                                reinterpret_cast<float2 *>(c->W), c->twf, f,
+                                result);
+    return rhs_gw_t<double2>(c, c->grad3, c->W, c->tw, f, result);
 }
 
 // mk_ffts_and_filter's filter branch (src/toolbox.c:164-171): phi, dphi,
