@@ -48,6 +48,9 @@ def parse():
     ap.add_argument("--skip", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--ensemble", action="store_true",
+                    help="only run the mixed-size ensemble (48 x 32^3 + 16 x "
+                         "64^3 + 4 x 128^3, RK4 dt 0.005) graph vs eager")
     ap.add_argument("--copy-probe", type=float, default=0.0, metavar="GB",
                     help="only time a plain copy kernel of GB gigabytes each "
                          "way (the HBM ceiling and the counter calibration)")
@@ -173,6 +176,35 @@ def run_gpu(args, wl):
     return dict(forms=forms, y0=y0, warm_t=w["t"])
 
 
+def ensemble_bench(steps):
+    """The batched form: realization-steps per second of the mixed ensemble
+    as one graph launch per step against eager launches, with the ensemble
+    mean and variance of the GW spectrum per size (graph run)."""
+    from instaheat_amd import ensemble
+    out = {}
+    for graph in (False, True):
+        r = ensemble.run_mixed(nsteps=steps, dt=0.005, use_graph=graph)
+        out["graph" if graph else "eager"] = r
+    g, e = out["graph"], out["eager"]
+    return {"metric": "realization-steps/s of the mixed ensemble "
+                      "(48 x 32^3 + 16 x 64^3 + 4 x 128^3, FP64 RK4 dt 0.005, GW on)",
+            "steps": steps,
+            "graph": {"value": round(g["realization_steps_per_s"], 2),
+                      "wall_s": round(g["wall_s"], 3), "rhs_evals": g["rhs_evals"]},
+            "eager": {"value": round(e["realization_steps_per_s"], 2),
+                      "wall_s": round(e["wall_s"], 3)},
+            "graph_over_eager": round(g["realization_steps_per_s"] /
+                                      e["realization_steps_per_s"], 3),
+            # the GW spectra agree to the rounding of their atomic bin sums
+            "graph_matches_eager": all(np.allclose(g["sizes"][n]["gw_mean"],
+                                                   e["sizes"][n]["gw_mean"],
+                                                   rtol=1e-12, atol=0)
+                                       for n in g["sizes"]),
+            "gw_spectrum": {str(n): {"nreal": v["nreal"], "t": v["t"],
+                                     "mean": v["gw_mean"], "var": v["gw_var"]}
+                            for n, v in g["sizes"].items()}}
+
+
 def measured_traffic(kernel, grid):
     """HBM bytes per launch of the dominant kernel from the committed counter
     summary (two rocprofv3 --pmc passes scaled by a copy kernel), or None."""
@@ -243,6 +275,9 @@ def main():
         sys.exit(2)
     if args.copy_probe > 0:
         print(json.dumps(copy_probe(args.copy_probe)))
+        return
+    if args.ensemble:
+        print(json.dumps(ensemble_bench(args.steps)))
         return
     wl = workload(args.grid)
     gpu = run_gpu(args, wl)

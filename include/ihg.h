@@ -141,6 +141,33 @@ int ihg_prof_read(ihg_ctx *c, double *ms, double *bytes, long *count,
 int ihg_prof_nfam(void);
 const char *ihg_prof_family(int fam);
 
+/* ---- ensembles (the batched form; no counterpart in the reference) ------
+ * An ensemble context holds nreal realizations of one grid: their states lie
+ * back to back in one device array, `stride` doubles apart (Ntot rounded up
+ * to even), and every kernel of ihg_mk_rhs_ensemble runs all of them in one
+ * launch (the realization is the third grid dimension).  ihg_rk4_stage /
+ * ihg_rk4_final on such a context step all nreal * stride values.  Calls on
+ * one realization's slice (ihg_mk_rhs, ihg_observables, ...) use the work
+ * arrays of realization 0. */
+int ihg_create_ensemble(const ihg_params *p, int nreal, ihg_ctx **out);
+/* out = {nreal, stride} */
+int ihg_ensemble(const ihg_ctx *c, long out[2]);
+/* mk_rhs of every realization: f, result are nreal * stride doubles;
+ * flags: IHG_FILTER only */
+int ihg_mk_rhs_ensemble(ihg_ctx *c, double t, double *f, double *result,
+                        unsigned flags);
+
+/* HIP graphs of a context's stream: capture a sequence of library calls
+ * that neither synchronises nor allocates (with the live profiler off),
+ * replay it as one launch; ihg_graph_join puts independent graphs (e.g. one
+ * per ensemble context) side by side in one graph. */
+typedef struct ihg_graph ihg_graph;
+int ihg_graph_begin(ihg_ctx *c);
+int ihg_graph_end(ihg_ctx *c, ihg_graph **g);
+int ihg_graph_join(ihg_graph *const *parts, int n, ihg_graph **g);
+int ihg_graph_launch(ihg_graph *g, ihg_ctx *c);
+int ihg_graph_destroy(ihg_graph *g);
+
 /* Switch the GW precision form (ihg_params.gw_fp32) for later calls. */
 int ihg_set_gw_fp32(ihg_ctx *c, int on);
 

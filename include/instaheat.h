@@ -157,6 +157,31 @@ void ih_proposer_features(const ih_dp_record *history, long n_history,
 double ih_mlp_propose(void *user, const ih_dp_record *history,
                       long n_history, double dt_classic);
 
+/* ---- ensembles: the batched form (no counterpart in the reference) ----
+ * nreal realizations of one grid in one ensemble context (ihg.h): fixed-step
+ * RK4 of run_rk4 (src/rk4.c:78-102) for all of them at once.  Several
+ * ensembles (one per grid size) step together in ih_ens_run_rk4; with
+ * use_graph each ensemble's step is captured once as a HIP graph and all of
+ * them are replayed side by side as one graph launch per step. */
+typedef struct ih_ens ih_ens;
+int ih_ens_create(const ih_config *cfg, int nreal, ih_ens **out);
+void ih_ens_destroy(ih_ens *e);
+ihg_ctx *ih_ens_gpu(ih_ens *e);
+double *ih_ens_state(ih_ens *e); /* nreal * stride doubles, device */
+long ih_ens_stride(const ih_ens *e);
+int ih_ens_nreal(const ih_ens *e);
+double ih_ens_time(const ih_ens *e);
+unsigned long ih_ens_calls_rhs(const ih_ens *e);
+/* Gaussian initial data per realization (ih_init_gaussian's recipe), then
+ * mk_initial_psi of each */
+int ih_ens_init(ih_ens *e, const uint64_t *seeds, const double *amplitudes,
+                double phi0, double dphi0, double k_cut);
+int ih_ens_capture_step(ih_ens *e, double dt, ihg_graph **g);
+int ih_ens_run_rk4(ih_ens *const *es, int n, long nsteps, double dt,
+                   int use_graph);
+/* the observables of realization r (an output mk_rhs on a copy of it) */
+int ih_ens_observables(ih_ens *e, int r, double *obs, size_t len);
+
 /* success() (src/dopri853.c:505-540) on a controller state, exported so its
  * arithmetic can be checked against the oracle's */
 typedef struct ih_ctrl {

@@ -129,7 +129,9 @@ IHG_SYMBOLS = [
     "ihg_rk4_stage", "ihg_rk4_final", "ihg_dp853_stage", "ihg_dp853_finish",
     "ihg_dp853_tableau", "ihg_aux", "ihg_prof_enable", "ihg_prof_reset",
     "ihg_prof_read", "ihg_prof_nfam", "ihg_prof_family", "ihg_features",
-    "ihg_set_gw_fp32",
+    "ihg_set_gw_fp32", "ihg_create_ensemble", "ihg_ensemble",
+    "ihg_mk_rhs_ensemble", "ihg_graph_begin", "ihg_graph_end",
+    "ihg_graph_join", "ihg_graph_launch", "ihg_graph_destroy",
     "ihg_mlp_nparams", "ihg_mlp_create", "ihg_mlp_destroy",
     "ihg_mlp_get_params", "ihg_mlp_set_params", "ihg_mlp_forward",
     "ihg_mlp_grad", "ihg_mlp_train",
@@ -140,7 +142,10 @@ HOST_SYMBOLS = [
     "ih_set_max_steps", "ih_set_tf",
     "ih_mk_rhs", "ih_init_gaussian", "ih_mk_initial_psi", "ih_run_rk4",
     "ih_run_dopri853", "ih_success", "ih_error_norm", "ih_proposer_features",
-    "ih_mlp_propose",
+    "ih_mlp_propose", "ih_ens_create", "ih_ens_destroy", "ih_ens_gpu",
+    "ih_ens_state", "ih_ens_stride", "ih_ens_nreal", "ih_ens_time",
+    "ih_ens_calls_rhs", "ih_ens_init", "ih_ens_capture_step",
+    "ih_ens_run_rk4", "ih_ens_observables",
 ]
 
 _lib = None
@@ -192,6 +197,15 @@ def lib():
         "ihg_prof_family": (ctypes.c_char_p, [i]),
         "ihg_features": (i, [_vp, _vp, _dptr]),
         "ihg_set_gw_fp32": (i, [_vp, i]),
+        "ihg_create_ensemble": (i, [ctypes.POINTER(IhgParams), i,
+                                    ctypes.POINTER(_vp)]),
+        "ihg_ensemble": (i, [_vp, ctypes.POINTER(ctypes.c_long)]),
+        "ihg_mk_rhs_ensemble": (i, [_vp, d, _vp, _vp, ctypes.c_uint]),
+        "ihg_graph_begin": (i, [_vp]),
+        "ihg_graph_end": (i, [_vp, ctypes.POINTER(_vp)]),
+        "ihg_graph_join": (i, [ctypes.POINTER(_vp), i, ctypes.POINTER(_vp)]),
+        "ihg_graph_launch": (i, [_vp, _vp]),
+        "ihg_graph_destroy": (i, [_vp]),
         "ihg_mlp_nparams": (i, []),
         "ihg_mlp_create": (i, [ctypes.POINTER(_vp), _fptr, ctypes.c_ulong]),
         "ihg_mlp_destroy": (i, [_vp]),
@@ -244,6 +258,18 @@ def host_lib():
                                             ctypes.POINTER(IhDpStats)]),
         "ih_success": (i, [ctypes.POINTER(IhCtrl), ctypes.POINTER(IhConfig), d,
                            _dptr, d]),
+        "ih_ens_create": (i, [ctypes.POINTER(IhConfig), i, ctypes.POINTER(_vp)]),
+        "ih_ens_destroy": (None, [_vp]),
+        "ih_ens_gpu": (_vp, [_vp]),
+        "ih_ens_state": (_vp, [_vp]),
+        "ih_ens_stride": (ctypes.c_long, [_vp]),
+        "ih_ens_nreal": (i, [_vp]),
+        "ih_ens_time": (d, [_vp]),
+        "ih_ens_calls_rhs": (ctypes.c_ulong, [_vp]),
+        "ih_ens_init": (i, [_vp, ctypes.POINTER(ctypes.c_uint64), _dptr, d, d, d]),
+        "ih_ens_capture_step": (i, [_vp, d, ctypes.POINTER(_vp)]),
+        "ih_ens_run_rk4": (i, [ctypes.POINTER(_vp), i, ctypes.c_long, d, i]),
+        "ih_ens_observables": (i, [_vp, i, _dptr, ctypes.c_size_t]),
         "ih_proposer_features": (None, [ctypes.POINTER(IhDpRecord),
                                         ctypes.c_long, _dptr]),
         "ih_mlp_propose": (d, [_vp, ctypes.POINTER(IhDpRecord), ctypes.c_long,

@@ -45,13 +45,14 @@ struct LineSet {
 // ---------------------------------------------------------------------------
 template <int L, int DIR, class C = double2>
 __global__ __launch_bounds__(FFT_WG) void k_c2c_strided(
-    const C *in, C *out, LineSet ls, long batch, const C *__restrict__ tw)
+    const C *in, C *out, LineSet ls, long batch, const C *__restrict__ tw,
+    long rr)
 {
     using TL = Tile<L>;
     __shared__ C lds[TL::B * TL::PITCH];
     const int tid = threadIdx.x;
-    in += blockIdx.y * batch;
-    out += blockIdx.y * batch;
+    in += blockIdx.y * batch + blockIdx.z * rr;
+    out += blockIdx.y * batch + blockIdx.z * rr;
     const int b = tid % TL::B, j = tid / TL::B;
     const long line = (long)blockIdx.x * TL::B + b;
     const bool act = line < ls.nlines;
@@ -74,7 +75,8 @@ __global__ __launch_bounds__(FFT_WG) void k_c2c_strided(
 // spectrum, each multiplied by its own factor fac(q, line, kx_index, value):
 // the spectral derivatives of the right-hand side (NQ = 4, out of place,
 // output q at out + q * qstride) and the filter round trip (NQ = 1, in
-// place, batched over blockIdx.y with arrays `batch` apart).
+// place, batched over blockIdx.y with arrays `batch` apart); realizations
+// of an ensemble over blockIdx.z, `rr` apart.
 // The spectrum never leaves registers: the forward transform's last stage
 // keeps its outputs, and each backward transform's first stage takes them
 // through chain_slot; HBM is read once and written NQ times.  With FWD =
@@ -85,13 +87,13 @@ template <int L, int NQ, bool FWD, class Fac, class P = Plan<L>,
           int WG = FFT_WG>
 __global__ __launch_bounds__(WG) void k_x_fanout(
     const double2 *in, double2 *out, long qstride, LineSet ls, long batch,
-    const double2 *__restrict__ tw, Fac fac)
+    const double2 *__restrict__ tw, Fac fac, long rr)
 {
     constexpr int E = P::E, T = L / E, B = WG / T, PITCH = Lds<L>::PITCH;
     __shared__ double2 lds[B * PITCH];
     const int tid = threadIdx.x;
-    in += blockIdx.y * batch;
-    out += blockIdx.y * batch;
+    in += blockIdx.y * batch + blockIdx.z * rr;
+    out += blockIdx.y * batch + blockIdx.z * rr;
     const int b = tid % B, j = tid / B;
     const long line = (long)blockIdx.x * B + b;
     const bool act = line < ls.nlines;
@@ -177,14 +179,14 @@ __device__ __forceinline__ double2 herm_merge(const double2 *__restrict__ A,
 template <int L>
 __global__ __launch_bounds__(FFT_WG) void k_z_r2c(
     const double *in, double2 *out, long nrows, long rbatch, long cbatch,
-    const double2 *__restrict__ tw)
+    const double2 *__restrict__ tw, long rin, long rout)
 {
     using TL = Tile<L>;
     constexpr int MZ = L / 2 + 1;
     __shared__ double2 lds[TL::B * TL::PITCH];
     const int tid = threadIdx.x;
-    in += blockIdx.y * rbatch;
-    out += blockIdx.y * cbatch;
+    in += blockIdx.y * rbatch + blockIdx.z * rin;
+    out += blockIdx.y * cbatch + blockIdx.z * rout;
     const long nlines = nrows / 2;
     const int bc = tid / TL::T;
     const long line = (long)blockIdx.x * TL::B + bc;
@@ -208,14 +210,14 @@ __global__ __launch_bounds__(FFT_WG) void k_z_r2c(
 template <int L>
 __global__ __launch_bounds__(FFT_WG) void k_z_c2r(
     const double2 *in, double *out, long nrows, long cbatch, long rbatch,
-    const double2 *__restrict__ tw)
+    const double2 *__restrict__ tw, long rin, long rout)
 {
     using TL = Tile<L>;
     constexpr int MZ = L / 2 + 1;
     __shared__ double2 lds[TL::B * TL::PITCH];
     const int tid = threadIdx.x;
-    in += blockIdx.y * cbatch;
-    out += blockIdx.y * rbatch;
+    in += blockIdx.y * cbatch + blockIdx.z * rin;
+    out += blockIdx.y * rbatch + blockIdx.z * rout;
     const long nlines = nrows / 2;
     const int bc = tid / TL::T;
     const long line = (long)blockIdx.x * TL::B + bc;

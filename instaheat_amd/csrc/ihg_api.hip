@@ -190,10 +190,12 @@ extern "C" int ihg_destroy(ihg_ctx *c)
     return 0;
 }
 
-extern "C" int ihg_create(const ihg_params *p, ihg_ctx **out)
+static int create_ctx(const ihg_params *p, int nreal, ihg_ctx **out)
 {
     if (!p || !out) return set_error("NULL argument");
     *out = nullptr;
+    if (nreal < 1 || nreal > 65535)
+        return set_error("ensemble of %d realizations (1..65535)", nreal);
     if (!pow2_in_range(p->nx) || !pow2_in_range(p->ny) || !pow2_in_range(p->nz))
         return set_error("grid %dx%dx%d: each size must be a power of two in "
                          "4..512", p->nx, p->ny, p->nz);
@@ -203,6 +205,7 @@ extern "C" int ihg_create(const ihg_params *p, ihg_ctx **out)
         return set_error("POWER_SPECTRUM_BINS must be in 1..2048");
     ihg_ctx *c = new ihg_ctx();
     c->p = *p;
+    c->nreal = nreal;
     c->Nx = p->nx;
     c->Ny = p->ny;
     c->Nz = p->nz;
@@ -275,16 +278,25 @@ extern "C" int ihg_create(const ihg_params *p, ihg_ctx **out)
         free(h);
         if (rc) return fail(rc);
     }
-    if ((rc = dalloc(&c->W, 6 * c->M))) return fail(rc);
+    // per-realization work arrays of an ensemble, one after the other
+    c->npartials = 3L * 512 * (p->bins > 8 ? p->bins : 8) + 2L * 65536 + 4096;
+    if (nreal > 1) {
+        c->ens.f = c->Ntot + 1; // even: every realization 16-byte aligned
+        c->ens.w = 6 * c->M;
+        c->ens.g = 3 * c->N;
+        c->ens.p = c->npartials;
+        c->ens.s = S_NSCAL;
+    }
+    if ((rc = dalloc(&c->W, 6 * c->M * nreal))) return fail(rc);
     if ((rc = dalloc(&c->Wc, 2 * c->M))) return fail(rc);
-    if (p->enable_gw && (rc = dalloc(&c->grad3, 3 * c->N))) return fail(rc);
+    if (p->enable_gw && (rc = dalloc(&c->grad3, 3 * c->N * nreal)))
+        return fail(rc);
     // rho | pressure | tmp.f in one allocation (rho and tmp.f 2N apart)
     if ((rc = dalloc(&c->rho, 3 * c->N))) return fail(rc);
     c->pressure = c->rho + c->N;
     c->tmpf = c->rho + 2 * c->N;
-    c->npartials = 3L * 512 * (p->bins > 8 ? p->bins : 8) + 2L * 65536 + 4096;
-    if ((rc = dalloc(&c->partials, c->npartials))) return fail(rc);
-    if ((rc = dalloc(&c->dscal, 16))) return fail(rc);
+    if ((rc = dalloc(&c->partials, c->npartials * nreal))) return fail(rc);
+    if ((rc = dalloc(&c->dscal, S_NSCAL * nreal))) return fail(rc);
     c->obs_len = 28 + 4L * p->bins;
     if ((rc = dalloc(&c->dobs, c->obs_len))) return fail(rc);
     if ((rc = hip_check(hipHostMalloc((void **)&c->hscal, 16 * sizeof(double)),
@@ -295,6 +307,25 @@ extern "C" int ihg_create(const ihg_params *p, ihg_ctx **out)
                         "hipHostMalloc")))
         return fail(rc);
     *out = c;
+    return 0;
+}
+
+extern "C" int ihg_create(const ihg_params *p, ihg_ctx **out)
+{
+    return create_ctx(p, 1, out);
+}
+
+extern "C" int ihg_create_ensemble(const ihg_params *p, int nreal,
+                                   ihg_ctx **out)
+{
+    return create_ctx(p, nreal, out);
+}
+
+extern "C" int ihg_ensemble(const ihg_ctx *c, long out[2])
+{
+    if (!c || !out) return set_error("NULL argument");
+    out[0] = c->nreal;
+    out[1] = c->nreal > 1 ? c->ens.f : c->Ntot;
     return 0;
 }
 
@@ -451,6 +482,26 @@ extern "C" int ihg_features(ihg_ctx *c, const double *f, double out[3])
     if (!c || !out) return set_error("NULL argument");
     IHG_TRY(check_ptr(f, "f"));
     return features(c, f, out);
+}
+
+extern "C" int ihg_mk_rhs_ensemble(ihg_ctx *c, double t, double *f,
+                                   double *result, unsigned flags)
+{
+    (void)t;
+    if (!c) return set_error("NULL context");
+    IHG_TRY(check_ptr(f, "f"));
+    IHG_TRY(check_ptr(result, "result"));
+    if (f == result) return set_error("f and result must differ");
+    if (flags & IHG_OUTPUT)
+        return set_error("ihg_mk_rhs_ensemble: output calls are per "
+                         "realization (ihg_mk_rhs on its slice)");
+    const int nr = c->nreal;
+    c->calls_rhs += nr;
+    const int kept = (flags & IHG_FILTER) && c->p.enable_filter;
+    if (kept) IHG_TRY(rhs_filter(c, f, nr));
+    IHG_TRY(rhs_core(c, f, result, 0, kept, nr));
+    if (c->p.enable_gw) IHG_TRY(rhs_gw(c, f, result, nr));
+    return 0;
 }
 
 extern "C" int ihg_means(ihg_ctx *c, double out[2])

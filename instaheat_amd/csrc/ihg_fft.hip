@@ -23,44 +23,51 @@ static LineSet x_lines(const ihg_ctx *c)
 
 template <int L, class C>
 static int launch_c2c(ihg_ctx *c, const C *in, C *out, LineSet ls, int dir,
-                      int batch, long cbatch, int fam, const C *tw)
+                      int batch, long cbatch, int fam, const C *tw, int nr,
+                      long rr)
 {
     using TL = Tile<L>;
-    dim3 grid((unsigned)((ls.nlines + TL::B - 1) / TL::B), batch);
+    dim3 grid((unsigned)((ls.nlines + TL::B - 1) / TL::B), batch, nr);
     // algorithmic bytes: every complex value read once and written once
-    const double bytes = 2.0 * sizeof(C) * c->M * batch;
+    const double bytes = 2.0 * sizeof(C) * c->M * batch * nr;
     return timed(c, fam, bytes, [&]() {
         if (dir < 0)
             hipLaunchKernelGGL((k_c2c_strided<L, -1, C>), grid, dim3(FFT_WG), 0,
-                               c->stream, in, out, ls, cbatch, tw);
+                               c->stream, in, out, ls, cbatch, tw, rr);
         else
             hipLaunchKernelGGL((k_c2c_strided<L, +1, C>), grid, dim3(FFT_WG), 0,
-                               c->stream, in, out, ls, cbatch, tw);
+                               c->stream, in, out, ls, cbatch, tw, rr);
         return hip_check(hipGetLastError(), "k_c2c_strided");
     });
 }
 
-int y_pass(ihg_ctx *c, double2 *data, int dir, int batch, long cbatch)
+int y_pass(ihg_ctx *c, double2 *data, int dir, int batch, long cbatch,
+           int nr, long rr)
 {
     return dispatch_len(c->Ny, [&](auto Lc) {
         return launch_c2c<decltype(Lc)::value>(c, data, data, y_lines(c), dir,
-                                               batch, cbatch, F_Y_PASS, c->tw);
+                                               batch, cbatch, F_Y_PASS, c->tw,
+                                               nr, rr);
     });
 }
 
-int x_pass(ihg_ctx *c, double2 *data, int dir, int batch, long cbatch)
+int x_pass(ihg_ctx *c, double2 *data, int dir, int batch, long cbatch,
+           int nr, long rr)
 {
     return dispatch_len(c->Nx, [&](auto Lc) {
         return launch_c2c<decltype(Lc)::value>(c, data, data, x_lines(c), dir,
-                                               batch, cbatch, F_X_PASS, c->tw);
+                                               batch, cbatch, F_X_PASS, c->tw,
+                                               nr, rr);
     });
 }
 
-int x_pass_f32(ihg_ctx *c, float2 *data, int dir, int batch, long cbatch)
+int x_pass_f32(ihg_ctx *c, float2 *data, int dir, int batch, long cbatch,
+               int nr, long rr)
 {
     return dispatch_len(c->Nx, [&](auto Lc) {
         return launch_c2c<decltype(Lc)::value>(c, data, data, x_lines(c), dir,
-                                               batch, cbatch, F_X_PASS, c->twf);
+                                               batch, cbatch, F_X_PASS, c->twf,
+                                               nr, rr);
     });
 }
 
@@ -69,39 +76,40 @@ static int x_pass_oop(ihg_ctx *c, const double2 *in, double2 *out, int dir,
 {
     return dispatch_len(c->Nx, [&](auto Lc) {
         return launch_c2c<decltype(Lc)::value>(c, in, out, x_lines(c), dir,
-                                               batch, cbatch, F_X_PASS, c->tw);
+                                               batch, cbatch, F_X_PASS, c->tw,
+                                               1, 0L);
     });
 }
 
 int z_r2c_pass(ihg_ctx *c, const double *in, double2 *out, int batch,
-               long rbatch, long cbatch)
+               long rbatch, long cbatch, int nr, long rin, long rout)
 {
     const long nrows = (long)c->Nx * c->Ny;
     return dispatch_len(c->Nz, [&](auto Lc) {
         constexpr int L = decltype(Lc)::value;
         using TL = Tile<L>;
-        dim3 grid((unsigned)((nrows / 2 + TL::B - 1) / TL::B), batch);
-        const double bytes = (8.0 * c->N + 16.0 * c->M) * batch;
+        dim3 grid((unsigned)((nrows / 2 + TL::B - 1) / TL::B), batch, nr);
+        const double bytes = (8.0 * c->N + 16.0 * c->M) * batch * nr;
         return timed(c, F_Z_R2C, bytes, [&]() {
             hipLaunchKernelGGL((k_z_r2c<L>), grid, dim3(FFT_WG), 0, c->stream,
-                               in, out, nrows, rbatch, cbatch, c->tw);
+                               in, out, nrows, rbatch, cbatch, c->tw, rin, rout);
             return hip_check(hipGetLastError(), "k_z_r2c");
         });
     });
 }
 
 int z_c2r_pass(ihg_ctx *c, const double2 *in, double *out, int batch,
-               long cbatch, long rbatch)
+               long cbatch, long rbatch, int nr, long rin, long rout)
 {
     const long nrows = (long)c->Nx * c->Ny;
     return dispatch_len(c->Nz, [&](auto Lc) {
         constexpr int L = decltype(Lc)::value;
         using TL = Tile<L>;
-        dim3 grid((unsigned)((nrows / 2 + TL::B - 1) / TL::B), batch);
-        const double bytes = (8.0 * c->N + 16.0 * c->M) * batch;
+        dim3 grid((unsigned)((nrows / 2 + TL::B - 1) / TL::B), batch, nr);
+        const double bytes = (8.0 * c->N + 16.0 * c->M) * batch * nr;
         return timed(c, F_Z_C2R, bytes, [&]() {
             hipLaunchKernelGGL((k_z_c2r<L>), grid, dim3(FFT_WG), 0, c->stream,
-                               in, out, nrows, cbatch, rbatch, c->tw);
+                               in, out, nrows, cbatch, rbatch, c->tw, rin, rout);
             return hip_check(hipGetLastError(), "k_z_c2r");
         });
     });

@@ -71,6 +71,19 @@ struct ProfRec {
 };
 } // namespace ihg
 
+namespace ihg {
+// Realization strides of an ensemble context: realization r of a batched
+// launch (blockIdx.z = r) finds its arrays r strides apart.  All zero for a
+// single realization.
+struct Ens {
+    long f = 0; // doubles between realizations' states (f, result)
+    long w = 0; // complex values between realizations' W arrays
+    long g = 0; // elements between realizations' gradients (grad3)
+    long p = 0; // doubles between realizations' reduction partials
+    long s = 0; // doubles between realizations' device scalars (dscal)
+};
+} // namespace ihg
+
 struct ihg_ctx {
     ihg_params p;
     int Nx, Ny, Nz, Mz;
@@ -94,6 +107,8 @@ struct ihg_ctx {
     double *hobs;      // pinned host copy of the observables
     long obs_len;
     double *field;     // array mk_summary reads (the reference's `field`)
+    int nreal;         // realizations of an ensemble context (1 otherwise)
+    ihg::Ens ens;      // their strides
     unsigned long calls_rhs;
     int have_obs;
     // live profiler: events around every launch of the timed families
@@ -112,13 +127,18 @@ int fft_r2c(ihg_ctx *c, const double *in, double2 *out, int batch,
             long rbatch, long cbatch);
 int fft_c2r(ihg_ctx *c, const double2 *in, double *out, double2 *work,
             int batch, long cbatch, long rbatch);
-int y_pass(ihg_ctx *c, double2 *data, int dir, int batch, long cbatch);
-int x_pass(ihg_ctx *c, double2 *data, int dir, int batch, long cbatch);
-int x_pass_f32(ihg_ctx *c, float2 *data, int dir, int batch, long cbatch);
+int y_pass(ihg_ctx *c, double2 *data, int dir, int batch, long cbatch,
+           int nr = 1, long rr = 0);
+int x_pass(ihg_ctx *c, double2 *data, int dir, int batch, long cbatch,
+           int nr = 1, long rr = 0);
+int x_pass_f32(ihg_ctx *c, float2 *data, int dir, int batch, long cbatch,
+               int nr = 1, long rr = 0);
 int z_r2c_pass(ihg_ctx *c, const double *in, double2 *out, int batch,
-               long rbatch, long cbatch);
+               long rbatch, long cbatch, int nr = 1, long rin = 0,
+               long rout = 0);
 int z_c2r_pass(ihg_ctx *c, const double2 *in, double *out, int batch,
-               long cbatch, long rbatch);
+               long cbatch, long rbatch, int nr = 1, long rin = 0,
+               long rout = 0);
 
 // right-hand side launchers (ihg_rhs.hip)
 // scalar slots in ctx->dscal (16 doubles)
@@ -132,10 +152,13 @@ enum {
     S_NSCAL = 16
 };
 
-int rhs_core(ihg_ctx *c, double *f, double *result, int output, int kept);
+// nr = 1: one realization (f, result, the work arrays of realization 0);
+// nr = c->nreal: every realization of an ensemble context at once
+int rhs_core(ihg_ctx *c, double *f, double *result, int output, int kept,
+             int nr = 1);
 int features(ihg_ctx *c, const double *f, double out[3]);
-int rhs_filter(ihg_ctx *c, double *f);
-int rhs_gw(ihg_ctx *c, const double *f, double *result);
+int rhs_filter(ihg_ctx *c, double *f, int nr = 1);
+int rhs_gw(ihg_ctx *c, const double *f, double *result, int nr = 1);
 int rhs_output(ihg_ctx *c, const double *f, int kept);
 int initial_psi(ihg_ctx *c, double *field, double *dfield);
 

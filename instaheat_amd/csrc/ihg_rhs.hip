@@ -64,6 +64,7 @@ struct FilterFac {
     long plane; // Ny * Mz
     double Nd;
     double2 *keep_phi, *keep_phi2, *keep_psi;
+    long rkeep; // realization stride of keep_phi (ensembles)
     __device__ double2 operator()(int, long line, int n, double2 v) const
     {
         const bool in = line < plane; // padding lines read entry 0
@@ -71,8 +72,8 @@ struct FilterFac {
         const double F = filt[i];
         const double2 w = make_double2(v.x * F, v.y * F);
         if (in && blockIdx.y == 0 && keep_phi) {
-            keep_phi[i] = w;
-            keep_phi2[i] = w;
+            keep_phi[blockIdx.z * rkeep + i] = w;
+            if (keep_phi2) keep_phi2[i] = w;
         }
         if (in && blockIdx.y == 2 && keep_psi)
             keep_psi[i] = w;
@@ -91,8 +92,14 @@ __global__ __launch_bounds__(FFT_WG) void k_z_c2r_rho(
     double *__restrict__ result, G *__restrict__ grad3,
     double *__restrict__ rho, double *__restrict__ pressure,
     double *__restrict__ partials, long nrows, long N, double m2,
-    const double2 *__restrict__ tw)
+    const double2 *__restrict__ tw, Ens e)
 {
+    // realization blockIdx.z of an ensemble
+    W += blockIdx.z * e.w;
+    f += blockIdx.z * e.f;
+    result += blockIdx.z * e.f;
+    partials += blockIdx.z * e.p;
+    if (grad3) grad3 += blockIdx.z * e.g;
     using TL = Tile<L>;
     constexpr int MZ = L / 2 + 1;
     constexpr int RPB = TL::B / 2; // rows per block
@@ -160,9 +167,14 @@ __global__ __launch_bounds__(FFT_WG) void k_z_c2r_rho(
 __global__ __launch_bounds__(RED_WG) void k_means(
     const double *__restrict__ partials, int nparts, long N,
     const double *__restrict__ f, double *__restrict__ result, long Ntot,
-    double *__restrict__ dscal)
+    double *__restrict__ dscal, Ens e)
 {
     __shared__ double sh[8];
+    // one work-group per realization
+    partials += blockIdx.x * e.p;
+    f += blockIdx.x * e.f;
+    result += blockIdx.x * e.f;
+    dscal += blockIdx.x * e.s;
     double s0 = 0.0, s1 = 0.0;
     for (int i = threadIdx.x; i < nparts; i += RED_WG) {
         s0 += partials[2 * i];
@@ -182,8 +194,11 @@ __global__ __launch_bounds__(RED_WG) void k_means(
 // result[N..2N) and result[3N..4N) by k_z_c2r_rho
 __global__ __launch_bounds__(RED_WG) void k_update_phi_psi(
     const double *__restrict__ f, double *__restrict__ result, long N,
-    long Ntot, double m2, const double *__restrict__ dscal)
+    long Ntot, double m2, const double *__restrict__ dscal, Ens e)
 {
+    f += blockIdx.z * e.f;
+    result += blockIdx.z * e.f;
+    dscal += blockIdx.z * e.s;
     const double aa = f[Ntot - 1];
     const double a2 = aa * aa;
     const double hubble = sqrt(dscal[S_RHO_MEAN] / 3.0);
@@ -256,8 +271,10 @@ struct GradRegs {
 template <int L, class C>
 __global__ __launch_bounds__(FFT_WG) void k_z_r2c_sources(
     const typename CT<C>::R *__restrict__ g3, C *__restrict__ W, long M,
-    long nrows, long N, const C *__restrict__ tw)
+    long nrows, long N, const C *__restrict__ tw, long rg, long rw)
 {
+    g3 += blockIdx.z * rg;
+    W += blockIdx.z * rw;
     using TL = Tile<L>;
     using R = typename CT<C>::R;
     __shared__ C lds[TL::B * TL::PITCH];
@@ -393,8 +410,14 @@ template <int L, int WG, class P, class C>
 __global__ __launch_bounds__(WG) void k_y_tt_update_h(
     const C *W, long M, const double *__restrict__ f,
     double *__restrict__ result, long N, long Next, long Ntot, KGeom kg,
-    const double *__restrict__ dscal, LineSet ls, const C *__restrict__ tw)
+    const double *__restrict__ dscal, LineSet ls, const C *__restrict__ tw,
+    Ens e, long rw)
 {
+    // realization blockIdx.z of an ensemble (rw: its W stride in units of C)
+    W += blockIdx.z * rw;
+    f += blockIdx.z * e.f;
+    result += blockIdx.z * e.f;
+    dscal += blockIdx.z * e.s;
     using R = typename CT<C>::R;
     constexpr int E = P::E, T = L / E, B = WG / T, PITCH = Lds<L>::PITCH;
     __shared__ C lds[B * PITCH];
@@ -476,17 +499,19 @@ static LineSet x_lineset(const ihg_ctx *c)
     return LineSet{plane, plane, 0, plane};
 }
 
-int rhs_core(ihg_ctx *c, double *f, double *result, int output, int kept)
+int rhs_core(ihg_ctx *c, double *f, double *result, int output, int kept,
+             int nr)
 {
     const long N = c->N, M = c->M;
     const double Nd = (double)N;
     const double m2 = c->p.mass * c->p.mass;
     double2 *W = c->W;
+    const Ens e = nr > 1 ? c->ens : Ens{};
     // phi_hat into W4: kept by the filter pass (full spectrum), or here by
     // the z and y passes (the x pass is the fan-out's first half)
     if (!kept) {
-        IHG_TRY(z_r2c_pass(c, f, W + 4 * M, 1, 0, 0));
-        IHG_TRY(y_pass(c, W + 4 * M, -1, 1, 0));
+        IHG_TRY(z_r2c_pass(c, f, W + 4 * M, 1, 0, 0, nr, e.f, e.w));
+        IHG_TRY(y_pass(c, W + 4 * M, -1, 1, 0, nr, e.w));
     }
     // x fan-out into the four derivative spectra W0..W3
     IHG_TRY(dispatch_len(c->Nx, [&](auto Lc) {
@@ -496,20 +521,20 @@ int rhs_core(ihg_ctx *c, double *f, double *result, int output, int kept)
         constexpr int B = FP::WG / (L / P::E);
         const LineSet ls = x_lineset(c);
         DerivFac fac{c->kg, Nd};
-        const dim3 grid((unsigned)((ls.nlines + B - 1) / B));
-        return timed(c, F_X_FANOUT, 5.0 * 16.0 * M, [&]() {
+        const dim3 grid((unsigned)((ls.nlines + B - 1) / B), 1, nr);
+        return timed(c, F_X_FANOUT, 5.0 * 16.0 * M * nr, [&]() {
             if (kept)
                 hipLaunchKernelGGL((k_x_fanout<L, 4, false, DerivFac, P, FP::WG>),
                                    grid, dim3(FP::WG), 0, c->stream, W + 4 * M,
-                                   W, M, ls, 0L, c->tw, fac);
+                                   W, M, ls, 0L, c->tw, fac, e.w);
             else
                 hipLaunchKernelGGL((k_x_fanout<L, 4, true, DerivFac, P, FP::WG>),
                                    grid, dim3(FP::WG), 0, c->stream, W + 4 * M,
-                                   W, M, ls, 0L, c->tw, fac);
+                                   W, M, ls, 0L, c->tw, fac, e.w);
             return hip_check(hipGetLastError(), "k_x_fanout");
         });
     }));
-    IHG_TRY(y_pass(c, W, +1, 4, M));
+    IHG_TRY(y_pass(c, W, +1, 4, M, nr, e.w));
     const long nrows = (long)c->Nx * c->Ny;
     const bool gw = c->p.enable_gw != 0;
     int nparts = 0;
@@ -522,63 +547,69 @@ int rhs_core(ihg_ctx *c, double *f, double *result, int output, int kept)
             return set_error("partials buffer too small");
         // 4 spectra in; phi, dphi, psi in; lap, p out (+ gradients, rho, p)
         const double gbytes = (gw && c->p.gw_fp32) ? 4.0 : 8.0;
-        const double bytes = 4.0 * 16.0 * M + 8.0 * N * (5 + (output ? 2 : 0)) +
-                             (gw ? 3.0 * gbytes * N : 0.0);
+        const double bytes = (4.0 * 16.0 * M + 8.0 * N * (5 + (output ? 2 : 0)) +
+                              (gw ? 3.0 * gbytes * N : 0.0)) * nr;
+        const dim3 grid(nb, 1, nr);
         return timed(c, F_Z_C2R_RHO, bytes, [&]() {
             if (gw && c->p.gw_fp32)
-                hipLaunchKernelGGL((k_z_c2r_rho<L, float>), dim3(nb),
+                hipLaunchKernelGGL((k_z_c2r_rho<L, float>), grid,
                                    dim3(FFT_WG), 0, c->stream, W, M, f, result,
                                    reinterpret_cast<float *>(c->grad3),
                                    output ? c->rho : nullptr,
                                    output ? c->pressure : nullptr, c->partials,
-                                   nrows, N, m2, c->tw);
+                                   nrows, N, m2, c->tw, e);
             else
-                hipLaunchKernelGGL((k_z_c2r_rho<L, double>), dim3(nb),
+                hipLaunchKernelGGL((k_z_c2r_rho<L, double>), grid,
                                    dim3(FFT_WG), 0, c->stream, W, M, f, result,
                                    gw ? c->grad3 : nullptr,
                                    output ? c->rho : nullptr,
                                    output ? c->pressure : nullptr, c->partials,
-                                   nrows, N, m2, c->tw);
+                                   nrows, N, m2, c->tw, e);
             return hip_check(hipGetLastError(), "k_z_c2r_rho");
         });
     }));
-    IHG_TRY(timed(c, F_MEANS, 16.0 * nparts, [&]() {
-        hipLaunchKernelGGL(k_means, dim3(1), dim3(RED_WG), 0, c->stream,
+    IHG_TRY(timed(c, F_MEANS, 16.0 * nparts * nr, [&]() {
+        hipLaunchKernelGGL(k_means, dim3(nr), dim3(RED_WG), 0, c->stream,
                            c->partials, nparts, N, f, result, c->Ntot,
-                           c->dscal);
+                           c->dscal, e);
         return hip_check(hipGetLastError(), "k_means");
     }));
-    return timed(c, F_UPDATE, 10.0 * 8.0 * N, [&]() {
+    return timed(c, F_UPDATE, 10.0 * 8.0 * N * nr, [&]() {
         hipLaunchKernelGGL(k_update_phi_psi,
-                           dim3(nblocks(N, RED_WG * 4, 65536)), dim3(RED_WG),
-                           0, c->stream, f, result, N, c->Ntot, m2, c->dscal);
+                           dim3(nblocks(N, RED_WG * 4, 65536), 1, nr),
+                           dim3(RED_WG), 0, c->stream, f, result, N, c->Ntot,
+                           m2, c->dscal, e);
         return hip_check(hipGetLastError(), "k_update_phi_psi");
     });
 }
 
 // GW source pipeline on complex type C: z r2c of the six products, x pass,
-// fused y pass + TT + update_h.  Wk holds 6 M values of C, g3 the gradients.
+// fused y pass + TT + update_h.  Wk holds 6 M values of C, g3 the gradients;
+// realization r of an ensemble finds them r * rw and r * rg further on.
 template <class C>
 static int rhs_gw_t(ihg_ctx *c, const typename CT<C>::R *g3, C *Wk,
-                    const C *tw, const double *f, double *result)
+                    const C *tw, const double *f, double *result, int nr,
+                    long rg, long rw)
 {
     const long N = c->N, M = c->M;
     const long nrows = (long)c->Nx * c->Ny;
     const double cs = sizeof(C), rs = sizeof(typename CT<C>::R);
+    const Ens e = nr > 1 ? c->ens : Ens{};
     IHG_TRY(dispatch_len(c->Nz, [&](auto Lc) {
         constexpr int L = decltype(Lc)::value;
         using TL = Tile<L>;
         const unsigned nb = (unsigned)((nrows + TL::B - 1) / TL::B);
-        return timed(c, F_Z_R2C_SRC, 3.0 * rs * N + 6.0 * cs * M, [&]() {
-            hipLaunchKernelGGL((k_z_r2c_sources<L, C>), dim3(nb), dim3(FFT_WG),
-                               0, c->stream, g3, Wk, M, nrows, N, tw);
+        return timed(c, F_Z_R2C_SRC, (3.0 * rs * N + 6.0 * cs * M) * nr, [&]() {
+            hipLaunchKernelGGL((k_z_r2c_sources<L, C>), dim3(nb, 1, nr),
+                               dim3(FFT_WG), 0, c->stream, g3, Wk, M, nrows, N,
+                               tw, rg, rw);
             return hip_check(hipGetLastError(), "k_z_r2c_sources");
         });
     }));
     if constexpr (std::is_same<C, float2>::value)
-        IHG_TRY(x_pass_f32(c, Wk, -1, 6, M));
+        IHG_TRY(x_pass_f32(c, Wk, -1, 6, M, nr, rw));
     else
-        IHG_TRY(x_pass(c, Wk, -1, 6, M));
+        IHG_TRY(x_pass(c, Wk, -1, 6, M, nr, rw));
     // y pass of the six sources, TT projection and update_h: six sources
     // and h1, h2, dh1, dh2 in; the four h blocks of result out
     return dispatch_len(c->Ny, [&](auto Lc) {
@@ -588,54 +619,60 @@ static int rhs_gw_t(ihg_ctx *c, const typename CT<C>::R *g3, C *Wk,
         // lines along y: one per (x, kz); points Mz apart
         const LineSet ls{(long)c->Nx * c->Mz, c->Mz, (long)c->Ny * c->Mz, c->Mz};
         const unsigned nb = (unsigned)((ls.nlines + B - 1) / B);
-        return timed(c, F_TT_UPDATE_H, (6.0 * cs + 8.0 * 16.0) * M, [&]() {
+        return timed(c, F_TT_UPDATE_H, (6.0 * cs + 8.0 * 16.0) * M * nr, [&]() {
             hipLaunchKernelGGL((k_y_tt_update_h<L, TP::WG, typename TP::P, C>),
-                               dim3(nb), dim3(TP::WG), 0, c->stream, Wk, M, f,
-                               result, N, c->Next, c->Ntot, c->kg, c->dscal,
-                               ls, tw);
+                               dim3(nb, 1, nr), dim3(TP::WG), 0, c->stream, Wk,
+                               M, f, result, N, c->Next, c->Ntot, c->kg,
+                               c->dscal, ls, tw, e, rw);
             return hip_check(hipGetLastError(), "k_y_tt_update_h");
         });
     });
 }
 
-int rhs_gw(ihg_ctx *c, const double *f, double *result)
+int rhs_gw(ihg_ctx *c, const double *f, double *result, int nr)
 {
+    const Ens e = nr > 1 ? c->ens : Ens{};
     if (c->p.gw_fp32)
         // FP32 values in the FP64 buffers: 6 M float2 fit in W, 3 N floats
-        // in grad3
+        // in grad3 (the same element strides per realization)
         return rhs_gw_t<float2>(c, reinterpret_cast<const float *>(c->grad3),
                                 reinterpret_cast<float2 *>(c->W), c->twf, f,
-                                result);
-    return rhs_gw_t<double2>(c, c->grad3, c->W, c->tw, f, result);
+                                result, nr, e.g, e.w);
+    return rhs_gw_t<double2>(c, c->grad3, c->W, c->tw, f, result, nr, e.g,
+                             e.w);
 }
 
 // mk_ffts_and_filter's filter branch (src/toolbox.c:164-171): phi, dphi,
 // psi, dpsi (contiguous in f) filtered in place, all four in one batch.
 // The filtered spectra of phi (-> W4 for the derivatives and Wc for the
 // output) and psi (-> Wc + M) are kept, as the reference keeps them.
-int rhs_filter(ihg_ctx *c, double *f)
+int rhs_filter(ihg_ctx *c, double *f, int nr)
 {
     const long N = c->N, M = c->M;
     double2 *W = c->W;
-    IHG_TRY(z_r2c_pass(c, f, W, 4, N, M));
-    IHG_TRY(y_pass(c, W, -1, 4, M));
+    const Ens e = nr > 1 ? c->ens : Ens{};
+    IHG_TRY(z_r2c_pass(c, f, W, 4, N, M, nr, e.f, e.w));
+    IHG_TRY(y_pass(c, W, -1, 4, M, nr, e.w));
     IHG_TRY(dispatch_len(c->Nx, [&](auto Lc) {
         constexpr int L = decltype(Lc)::value;
         using TL = Tile<L>;
         const LineSet ls = x_lineset(c);
+        // one realization: keep phi_hat in W4 and Wc, psi_hat in Wc + M;
+        // an ensemble (no output calls) keeps phi_hat in each W4
         FilterFac fac{c->filter, (long)c->Ny * c->Mz, (double)N, W + 4 * M,
-                      c->Wc, c->Wc + M};
+                      nr > 1 ? nullptr : c->Wc,
+                      nr > 1 ? nullptr : c->Wc + M, e.w};
         // four fields in and out, three kept spectra out
-        return timed(c, F_X_ROUNDTRIP, 11.0 * 16.0 * M, [&]() {
+        return timed(c, F_X_ROUNDTRIP, 11.0 * 16.0 * M * nr, [&]() {
             hipLaunchKernelGGL((k_x_fanout<L, 1, true, FilterFac>),
-                               dim3((unsigned)((ls.nlines + TL::B - 1) / TL::B), 4),
+                               dim3((unsigned)((ls.nlines + TL::B - 1) / TL::B), 4, nr),
                                dim3(FFT_WG), 0, c->stream, W, W, 0L, ls, M,
-                               c->tw, fac);
+                               c->tw, fac, e.w);
             return hip_check(hipGetLastError(), "k_x_fanout (filter)");
         });
     }));
-    IHG_TRY(y_pass(c, W, +1, 4, M));
-    return z_c2r_pass(c, W, f, 4, M, N);
+    IHG_TRY(y_pass(c, W, +1, 4, M, nr, e.w));
+    return z_c2r_pass(c, W, f, 4, M, N, nr, e.w, e.f);
 }
 
 } // namespace ihg

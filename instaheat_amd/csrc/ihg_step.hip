@@ -26,6 +26,19 @@ static inline long live_len(const ihg_ctx *c)
     return c->p.enable_gw ? c->Ntot - 1 : 4 * c->N;
 }
 
+// RK4 vectors of a context: one state (live part + a) or, for an ensemble,
+// all realizations' states back to back (nreal * stride, every element)
+static inline void rk4_extent(const ihg_ctx *c, long &nl, long &tail)
+{
+    if (c->nreal > 1) {
+        nl = c->nreal * c->ens.f;
+        tail = -1;
+    } else {
+        nl = live_len(c);
+        tail = c->Ntot - 1;
+    }
+}
+
 static inline unsigned stream_blocks(long npairs)
 {
     long b = (npairs + RED_WG - 1) / RED_WG;
@@ -66,10 +79,11 @@ __global__ __launch_bounds__(RED_WG) void k_dp_stage(
     }
 }
 
-// RK4 stage: out = y + dt * k / 2 (half) or y + dt * k
+// RK4 stage: out = y + dt * k / 2 (half) or y + dt * k, over the pairs of
+// [0, nlive) and the element `tail` (the scale factor; -1: none)
 __global__ __launch_bounds__(RED_WG) void k_rk4_stage(
     double *__restrict__ out, const double *__restrict__ y,
-    const double *__restrict__ k, double dt, int half, long nlive, long Ntot)
+    const double *__restrict__ k, double dt, int half, long nlive, long tail)
 {
     const long npairs = nlive / 2;
     const long stride = (long)gridDim.x * RED_WG;
@@ -84,8 +98,8 @@ __global__ __launch_bounds__(RED_WG) void k_rk4_stage(
         else
             o2[i] = make_double2(yy.x + dt * kk.x, yy.y + dt * kk.y);
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        const long i = Ntot - 1;
+    if (tail >= 0 && blockIdx.x == 0 && threadIdx.x == 0) {
+        const long i = tail;
         out[i] = half ? y[i] + dt * k[i] / 2.0 : y[i] + dt * k[i];
     }
 }
@@ -94,7 +108,7 @@ __global__ __launch_bounds__(RED_WG) void k_rk4_stage(
 __global__ __launch_bounds__(RED_WG) void k_rk4_final(
     double *__restrict__ y, const double *__restrict__ k1,
     const double *__restrict__ k2, const double *__restrict__ k3,
-    const double *__restrict__ k4, double dt, long nlive, long Ntot)
+    const double *__restrict__ k4, double dt, long nlive, long tail)
 {
     const long npairs = nlive / 2;
     const long stride = (long)gridDim.x * RED_WG;
@@ -110,8 +124,8 @@ __global__ __launch_bounds__(RED_WG) void k_rk4_final(
         yy.y += dt * (a.y + 2.0 * b.y + 2.0 * c.y + d.y) / 6.0;
         y2[i] = yy;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        const long i = Ntot - 1;
+    if (tail >= 0 && blockIdx.x == 0 && threadIdx.x == 0) {
+        const long i = tail;
         y[i] += dt * (k1[i] + 2.0 * k2[i] + 2.0 * k3[i] + k4[i]) / 6.0;
     }
 }
@@ -208,11 +222,12 @@ extern "C" int ihg_rk4_stage(ihg_ctx *c, double *out, const double *y,
     IHG_TRY(check_ptr(out, "out"));
     IHG_TRY(check_ptr(y, "y"));
     IHG_TRY(check_ptr(k, "k"));
-    const long nl = live_len(c);
+    long nl, tail;
+    rk4_extent(c, nl, tail);
     return timed(c, F_RK4, 3.0 * 8.0 * nl, [&]() {
         hipLaunchKernelGGL(k_rk4_stage, dim3(stream_blocks(nl / 2)),
                            dim3(RED_WG), 0, c->stream, out, y, k, dt,
-                           half ? 1 : 0, nl, c->Ntot);
+                           half ? 1 : 0, nl, tail);
         return hip_check(hipGetLastError(), "k_rk4_stage");
     });
 }
@@ -227,11 +242,12 @@ extern "C" int ihg_rk4_final(ihg_ctx *c, double *y, const double *k1,
     IHG_TRY(check_ptr(k2, "k2"));
     IHG_TRY(check_ptr(k3, "k3"));
     IHG_TRY(check_ptr(k4, "k4"));
-    const long nl = live_len(c);
+    long nl, tail;
+    rk4_extent(c, nl, tail);
     return timed(c, F_RK4, 6.0 * 8.0 * nl, [&]() {
         hipLaunchKernelGGL(k_rk4_final, dim3(stream_blocks(nl / 2)),
                            dim3(RED_WG), 0, c->stream, y, k1, k2, k3, k4, dt,
-                           nl, c->Ntot);
+                           nl, tail);
         return hip_check(hipGetLastError(), "k_rk4_final");
     });
 }

@@ -196,18 +196,22 @@ static double unit_open0(uint64_t r)
     return ((double)(r >> 11) + 1.0) * (1.0 / 9007199254740992.0);
 }
 
-int ih_init_gaussian(ih_sim *s, uint64_t seed, double amplitude, double phi0,
-                     double dphi0, double k_cut)
+/* Gaussian initial data (the build's seeded recipe) into the device state
+ * `field` (Ntot doubles) of a grid; `work` holds at least 2M doubles. */
+static int gaussian_into(ihg_ctx *g, const ih_config *cfg, size_t N, size_t M,
+                         size_t Ntot, double *field, double *work,
+                         uint64_t seed, double amplitude, double phi0,
+                         double dphi0, double k_cut)
 {
-    const ihg_params *p = &s->cfg.grid;
+    const ihg_params *p = &cfg->grid;
     const int Nx = p->nx, Ny = p->ny, Nz = p->nz, Mz = Nz / 2 + 1;
     const double TWOPI = 6.283185307179586476925286766559;
     const double kx1 = TWOPI / (p->xb - p->xa), ky1 = TWOPI / (p->yb - p->ya),
                  kz1 = TWOPI / (p->zb - p->za);
-    const double a0 = s->cfg.a_initial;
+    const double a0 = cfg->a_initial;
     const double m2 = p->mass * p->mass;
-    double *spec = calloc(2 * s->M, sizeof(double));
-    double *real = calloc(s->Ntot, sizeof(double));
+    double *spec = calloc(2 * M, sizeof(double));
+    double *real = calloc(Ntot, sizeof(double));
     if (!spec || !real) {
         free(spec);
         free(real);
@@ -234,22 +238,31 @@ int ih_init_gaussian(ih_sim *s, uint64_t seed, double amplitude, double phi0,
             }
         }
     }
-    int rc = ihg_zero(s->g, s->field, s->Ntot);
-    if (!rc) rc = ihg_zero(s->g, s->dfield, s->Ntot);
-    if (!rc) rc = ihg_upload(s->g, s->k_tmp, spec, 2 * s->M);
-    if (!rc) rc = ihg_fft_c2r(s->g, s->k_tmp, s->field);
-    if (!rc) rc = ihg_download(s->g, real, s->field, s->N);
+    int rc = ihg_zero(g, field, Ntot);
+    if (!rc) rc = ihg_upload(g, work, spec, 2 * M);
+    if (!rc) rc = ihg_fft_c2r(g, work, field);
+    if (!rc) rc = ihg_download(g, real, field, N);
     if (!rc) {
-        for (size_t i = 0; i < s->N; ++i) {
+        for (size_t i = 0; i < N; ++i) {
             real[i] += phi0;
-            real[s->N + i] = dphi0;
+            real[N + i] = dphi0;
         }
-        real[s->Ntot - 1] = a0;
-        rc = ihg_upload(s->g, s->field, real, s->Ntot);
+        real[Ntot - 1] = a0;
+        rc = ihg_upload(g, field, real, Ntot);
     }
-    if (!rc) rc = ihg_zero(s->g, s->k_tmp, s->Ntot);
+    if (!rc) rc = ihg_zero(g, work, 2 * M);
     free(spec);
     free(real);
+    return rc;
+}
+
+int ih_init_gaussian(ih_sim *s, uint64_t seed, double amplitude, double phi0,
+                     double dphi0, double k_cut)
+{
+    int rc = ihg_zero(s->g, s->dfield, s->Ntot);
+    if (!rc)
+        rc = gaussian_into(s->g, &s->cfg, s->N, s->M, s->Ntot, s->field,
+                           s->k_tmp, seed, amplitude, phi0, dphi0, k_cut);
     s->t = s->cfg.ti;
     return rc;
 }
@@ -551,4 +564,176 @@ double ih_mlp_propose(void *user, const ih_dp_record *history, long n,
     if (lr > hi) lr = hi;
     p->calls++;
     return history[n - 1].dt * exp(lr);
+}
+
+/* ------------------------------------------------------------------------ */
+/* ensembles: fixed-step RK4 of many realizations of one grid               */
+/* ------------------------------------------------------------------------ */
+
+struct ih_ens {
+    ih_config cfg;
+    ihg_ctx *g;
+    int nreal;
+    long stride;            /* doubles between realizations' states */
+    size_t N, M, Next, Ntot;
+    double *y, *k[4], *tmp; /* nreal * stride doubles each, device */
+    double t;
+    unsigned long calls;    /* RHS evaluations of all realizations */
+};
+
+int ih_ens_create(const ih_config *cfg, int nreal, ih_ens **out)
+{
+    if (!cfg || !out || nreal < 1) return IHG_EINVAL;
+    *out = NULL;
+    ih_ens *e = calloc(1, sizeof *e);
+    if (!e) return IHG_ENOMEM;
+    e->cfg = *cfg;
+    e->nreal = nreal;
+    int rc = ihg_create_ensemble(&cfg->grid, nreal, &e->g);
+    if (rc) {
+        free(e);
+        return rc;
+    }
+    size_t sz[4];
+    long ens[2];
+    ihg_sizes(e->g, sz);
+    ihg_ensemble(e->g, ens);
+    e->N = sz[0];
+    e->M = sz[1];
+    e->Next = sz[2];
+    e->Ntot = sz[3];
+    e->stride = ens[1];
+    const size_t n = (size_t)nreal * e->stride;
+    rc = ihg_malloc(e->g, &e->y, n);
+    for (int i = 0; !rc && i < 4; ++i) rc = ihg_malloc(e->g, &e->k[i], n);
+    if (!rc) rc = ihg_malloc(e->g, &e->tmp, n);
+    if (rc) {
+        ih_ens_destroy(e);
+        return rc;
+    }
+    e->t = cfg->ti;
+    *out = e;
+    return 0;
+}
+
+void ih_ens_destroy(ih_ens *e)
+{
+    if (!e) return;
+    if (e->g) {
+        ihg_free(e->g, e->y);
+        for (int i = 0; i < 4; ++i) ihg_free(e->g, e->k[i]);
+        ihg_free(e->g, e->tmp);
+        ihg_destroy(e->g);
+    }
+    free(e);
+}
+
+ihg_ctx *ih_ens_gpu(ih_ens *e) { return e->g; }
+double *ih_ens_state(ih_ens *e) { return e->y; }
+long ih_ens_stride(const ih_ens *e) { return e->stride; }
+int ih_ens_nreal(const ih_ens *e) { return e->nreal; }
+double ih_ens_time(const ih_ens *e) { return e->t; }
+unsigned long ih_ens_calls_rhs(const ih_ens *e) { return e->calls; }
+
+int ih_ens_init(ih_ens *e, const uint64_t *seeds, const double *amplitudes,
+                double phi0, double dphi0, double k_cut)
+{
+    int rc = 0;
+    for (int r = 0; !rc && r < e->nreal; ++r) {
+        double *f = e->y + (size_t)r * e->stride;
+        rc = gaussian_into(e->g, &e->cfg, e->N, e->M, e->Ntot, f, e->tmp,
+                           seeds[r], amplitudes[r], phi0, dphi0, k_cut);
+        /* mk_initial_psi (src/setup.c:825-903) of this realization */
+        if (!rc) rc = ihg_set_field(e->g, f);
+        if (!rc) rc = ihg_mk_initial_psi(e->g, e->cfg.ti, f, e->k[0]);
+        e->calls += 1;
+    }
+    if (!rc) rc = ihg_set_field(e->g, NULL);
+    e->t = e->cfg.ti;
+    return rc;
+}
+
+/* one step of run_rk4 (src/rk4.c:78-102) for every realization; the filter
+ * acts at the first stage, as there */
+static int ens_step(ih_ens *e, double t, double dt)
+{
+    ihg_ctx *g = e->g;
+    const unsigned f1 = e->cfg.grid.enable_filter ? IHG_FILTER : 0u;
+    int rc = ihg_mk_rhs_ensemble(g, t, e->y, e->k[0], f1);
+    if (!rc) rc = ihg_rk4_stage(g, e->tmp, e->y, e->k[0], dt, 1);
+    if (!rc) rc = ihg_mk_rhs_ensemble(g, t + dt / 2.0, e->tmp, e->k[1], 0);
+    if (!rc) rc = ihg_rk4_stage(g, e->tmp, e->y, e->k[1], dt, 1);
+    if (!rc) rc = ihg_mk_rhs_ensemble(g, t + dt / 2.0, e->tmp, e->k[2], 0);
+    if (!rc) rc = ihg_rk4_stage(g, e->tmp, e->y, e->k[2], dt, 0);
+    if (!rc) rc = ihg_mk_rhs_ensemble(g, t + dt, e->tmp, e->k[3], 0);
+    if (!rc)
+        rc = ihg_rk4_final(g, e->y, e->k[0], e->k[1], e->k[2], e->k[3], dt);
+    return rc;
+}
+
+int ih_ens_capture_step(ih_ens *e, double dt, ihg_graph **out)
+{
+    int rc = ihg_graph_begin(e->g);
+    if (rc) return rc;
+    rc = ens_step(e, e->t, dt);
+    ihg_graph *g = NULL;
+    const int rc2 = ihg_graph_end(e->g, &g);
+    if (rc || rc2) {
+        ihg_graph_destroy(g);
+        return rc ? rc : rc2;
+    }
+    *out = g;
+    return 0;
+}
+
+int ih_ens_run_rk4(ih_ens *const *es, int n, long nsteps, double dt,
+                   int use_graph)
+{
+    if (!es || n < 1 || nsteps < 0) return IHG_EINVAL;
+    int rc = 0;
+    if (!use_graph) {
+        for (long s = 0; !rc && s < nsteps; ++s)
+            for (int i = 0; !rc && i < n; ++i) {
+                rc = ens_step(es[i], es[i]->t, dt);
+                es[i]->t += dt;
+                es[i]->calls += 4UL * es[i]->nreal;
+            }
+    } else {
+        /* one graph per ensemble, all of them side by side in one graph:
+           every step of every realization of every size is one launch */
+        ihg_graph **parts = calloc(n, sizeof *parts);
+        ihg_graph *all = NULL;
+        if (!parts) return IHG_ENOMEM;
+        for (int i = 0; !rc && i < n; ++i)
+            rc = ih_ens_capture_step(es[i], dt, &parts[i]);
+        if (!rc) rc = ihg_graph_join(parts, n, &all);
+        for (long s = 0; !rc && s < nsteps; ++s) {
+            rc = ihg_graph_launch(all, es[0]->g);
+            for (int i = 0; i < n; ++i) {
+                es[i]->t += dt;
+                es[i]->calls += 4UL * es[i]->nreal;
+            }
+        }
+        if (!rc) rc = ihg_sync(es[0]->g);
+        ihg_graph_destroy(all);
+        for (int i = 0; i < n; ++i) ihg_graph_destroy(parts[i]);
+        free(parts);
+    }
+    for (int i = 0; !rc && i < n; ++i) rc = ihg_sync(es[i]->g);
+    return rc;
+}
+
+int ih_ens_observables(ih_ens *e, int r, double *obs, size_t len)
+{
+    if (r < 0 || r >= e->nreal) return IHG_EINVAL;
+    double *f = e->y + (size_t)r * e->stride;
+    /* prepare_and_save_timeslice (src/toolbox.c:96-104) on a copy, so the
+       filter of the output call leaves the ensemble state alone */
+    int rc = ihg_copy(e->g, e->tmp, f, e->Ntot);
+    if (!rc) rc = ihg_set_field(e->g, e->tmp);
+    if (!rc) rc = ihg_mk_rhs(e->g, e->t, e->tmp, e->k[0], IHG_FILTER | IHG_OUTPUT);
+    if (!rc) rc = ihg_observables(e->g, obs, len);
+    if (!rc) rc = ihg_set_field(e->g, NULL);
+    e->calls += 1;
+    return rc;
 }

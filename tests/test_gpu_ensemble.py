@@ -1,0 +1,144 @@
+"""GPU checks of the batched form: an ensemble context's one-launch RHS of
+every realization equals the single-realization RHS of each, its RK4 steps
+equal single-realization RK4 (run_rk4), and the HIP-graph replay of a step
+equals the eager step after every replay."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from tests.states import BOX_CONFIG, make_state
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ih():
+    import instaheat_amd
+    instaheat_amd.host_lib()
+    return instaheat_amd
+
+
+@pytest.fixture(scope="module")
+def E():
+    from instaheat_amd import ensemble
+    return ensemble
+
+
+@pytest.mark.parametrize("shape,gw,flags,fp32", [((16, 16, 16), True, 0, False),
+                                                 ((16, 8, 32), True, 1, False),
+                                                 ((32, 32, 32), False, 1, False),
+                                                 ((16, 16, 16), True, 1, True)])
+def test_ensemble_rhs_equals_single_realizations(ih, shape, gw, flags, fp32):
+    nreal = 3
+    p = ih.make_params(shape, BOX_CONFIG, 4.0, gw, True, 20, fp32)
+    h = ctypes.c_void_p()
+    ih._check(ih.lib().ihg_create_ensemble(ctypes.byref(p), nreal, ctypes.byref(h)))
+    ctx = h.value
+    ens = (ctypes.c_long * 2)()
+    ih._check(ih.lib().ihg_ensemble(ctx, ens))
+    stride = ens[1]
+    g = ih.Gpu(shape, bounds=BOX_CONFIG, mass=4.0, gw=gw, bins=20, gw_fp32=fp32)
+    ys = [make_state(shape, seed=40 + r, gw=gw, a=1.1 + 0.2 * r) for r in range(nreal)]
+    full = np.zeros((nreal, stride))
+    for r, y in enumerate(ys):
+        full[r, :y.size] = y
+    L = ih.lib()
+    n = nreal * stride
+    fd = ctypes.c_void_p()
+    rd = ctypes.c_void_p()
+    ih._check(L.ihg_malloc(ctx, ctypes.byref(fd), n))
+    ih._check(L.ihg_malloc(ctx, ctypes.byref(rd), n))
+    ih._check(L.ihg_upload(ctx, fd, full.ctypes.data_as(ctypes.c_void_p), n))
+    ih._check(L.ihg_mk_rhs_ensemble(ctx, 0.0, fd, rd, flags))
+    got = np.empty(n)
+    ih._check(L.ihg_download(ctx, got.ctypes.data_as(ctypes.c_void_p), rd, n))
+    filtered = np.empty(n)
+    ih._check(L.ihg_download(ctx, filtered.ctypes.data_as(ctypes.c_void_p), fd, n))
+    got = got.reshape(nreal, stride)
+    filtered = filtered.reshape(nreal, stride)
+    for r, y in enumerate(ys):
+        f = g.to_device(y)
+        res = g.alloc(g.Ntot)
+        g.mk_rhs(f, res, flags=flags)
+        assert np.array_equal(got[r, :y.size], res.download()), r
+        assert np.array_equal(filtered[r, :y.size], f.download()), r
+    L.ihg_free(ctx, fd)
+    L.ihg_free(ctx, rd)
+    L.ihg_destroy(ctx)
+
+
+def test_ensemble_rk4_equals_run_rk4(ih, E):
+    """Realization r of an ensemble stepped by RK4 equals run_rk4 of one
+    simulation from the same initial data, bit for bit.  (Filter off on both
+    sides: run_rk4's final prepare_and_save filters the state once more.)"""
+    n, nreal, steps, dt = 16, 3, 6, 0.01
+    e = E.Ensemble(n, nreal, kmin=0.1, bins=16, filt=False)
+    seeds, amps = [7, 8, 9], [1e-5, 3e-5, 1e-4]
+    e.init(seeds, amps)
+    E.run([e], steps, dt, use_graph=False)
+    ens_end = e.state()
+    L = 2 * np.pi / 0.1
+    for r in range(nreal):
+        # max_steps rather than tf, so that no last step is shortened
+        sim = ih.Sim((n, n, n), bounds=((0.0, L),) * 3, mass=1.0, gw=True,
+                     filt=False, bins=16, method=ih.IH_RK4, tf=1.0,
+                     max_steps=steps, dt=dt, skip=10**9)
+        sim.init_gaussian(seed=seeds[r], amplitude=amps[r], k_cut=n * 0.1 / 4)
+        sim.mk_initial_psi()
+        sim.run_rk4()
+        assert np.array_equal(sim.field(), ens_end[r]), r
+    e.close()
+
+
+def test_graph_replay_equals_eager_steps(ih, E):
+    """Two ensembles (16^3 x 3, 32^3 x 2) in one joined graph: after every
+    replay, with eager work and synchronisation in between, the states equal
+    the eager steps exactly."""
+    def make():
+        a = E.Ensemble(16, 3, bins=16)
+        a.init([1, 2, 3], [1e-5, 2e-5, 4e-5])
+        b = E.Ensemble(32, 2, bins=16)
+        b.init([4, 5], [1e-4, 1e-6])
+        return a, b
+    ea, eb = make()
+    ga, gb = make()
+    dt = 0.005
+    H, L = ih.host_lib(), ih.lib()
+    pa, pb = ctypes.c_void_p(), ctypes.c_void_p()
+    ih._check(H.ih_ens_capture_step(ga.h, dt, ctypes.byref(pa)))
+    ih._check(H.ih_ens_capture_step(gb.h, dt, ctypes.byref(pb)))
+    parts = (ctypes.c_void_p * 2)(pa.value, pb.value)
+    joined = ctypes.c_void_p()
+    ih._check(L.ihg_graph_join(parts, 2, ctypes.byref(joined)))
+    for step in range(4):
+        E.run([ea, eb], 1, dt, use_graph=False)
+        ih._check(L.ihg_graph_launch(joined, ga.ctx))
+        ih._check(L.ihg_sync(ga.ctx))
+        # eager work of the program itself between the replays
+        _ = ea.observables(0)
+        assert np.array_equal(ga.state(), ea.state()), step
+        assert np.array_equal(gb.state(), eb.state()), step
+    L.ihg_graph_destroy(joined)
+    for p in parts:
+        L.ihg_graph_destroy(p)
+    for e in (ea, eb, ga, gb):
+        e.close()
+
+
+def test_mixed_ensemble_short(E):
+    """The mixed-size ensemble machinery at reduced size: graph and eager
+    stepping agree (the states exactly; the GW spectra to the rounding of
+    their atomic bin sums), and the GW spectrum statistics are finite."""
+    sizes = ((16, 4), (32, 2))
+    a = E.run_mixed(nsteps=5, dt=0.005, use_graph=True, sizes=sizes, bins=16,
+                    keep_states=True)
+    b = E.run_mixed(nsteps=5, dt=0.005, use_graph=False, sizes=sizes, bins=16,
+                    keep_states=True)
+    for n, _ in sizes:
+        assert np.array_equal(a["sizes"][n]["states"], b["sizes"][n]["states"])
+        assert np.allclose(a["sizes"][n]["gw_mean"], b["sizes"][n]["gw_mean"],
+                           rtol=1e-12, atol=0)
+        assert np.all(np.isfinite(a["sizes"][n]["gw_var"]))
+        assert a["sizes"][n]["t"] == pytest.approx(0.025)
+    assert a["rhs_evals"] == 4 * 5 * 6
