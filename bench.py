@@ -182,10 +182,11 @@ def ensemble_bench(steps):
     mean and variance of the GW spectrum per size (graph run)."""
     from instaheat_amd import ensemble
     out = {}
-    for graph in (False, True):
-        r = ensemble.run_mixed(nsteps=steps, dt=0.005, use_graph=graph)
-        out["graph" if graph else "eager"] = r
-    g, e = out["graph"], out["eager"]
+    for name, graph, tune in (("eager", False, False), ("graph", True, False),
+                              ("tuned", True, True)):
+        out[name] = ensemble.run_mixed(nsteps=steps, dt=0.005, use_graph=graph,
+                                       autotune=tune)
+    g, e, t = out["graph"], out["eager"], out["tuned"]
     return {"metric": "realization-steps/s of the mixed ensemble "
                       "(48 x 32^3 + 16 x 64^3 + 4 x 128^3, FP64 RK4 dt 0.005, GW on)",
             "steps": steps,
@@ -194,6 +195,11 @@ def ensemble_bench(steps):
             "eager": {"value": round(e["realization_steps_per_s"], 2),
                       "wall_s": round(e["wall_s"], 3)},
             "graph_over_eager": round(g["realization_steps_per_s"] /
+                                      e["realization_steps_per_s"], 3),
+            "tuned": {"value": round(t["realization_steps_per_s"], 2),
+                      "wall_s": round(t["wall_s"], 3),
+                      "candidates_yx": t["autotuned"]},
+            "tuned_over_eager": round(t["realization_steps_per_s"] /
                                       e["realization_steps_per_s"], 3),
             # the GW spectra agree to the rounding of their atomic bin sums
             "graph_matches_eager": all(np.allclose(g["sizes"][n]["gw_mean"],

@@ -168,6 +168,20 @@ int ihg_graph_join(ihg_graph *const *parts, int n, ihg_graph **g);
 int ihg_graph_launch(ihg_graph *g, ihg_ctx *c);
 int ihg_graph_destroy(ihg_graph *g);
 
+/* Autotuner of the strided y and x passes (no counterpart in the
+ * reference): every candidate line plan and work-group size for the y and
+ * the x line length is timed `reps` times on the context's work arrays (in
+ * the RHS's shape: four arrays, every realization of an ensemble) and the
+ * fastest is kept for later calls.  choice = {y, x} candidate indices (0 is
+ * the default); ms (may be NULL) receives up to nms average times, y
+ * candidates first.  Candidates differ in rounding only. */
+int ihg_autotune(ihg_ctx *c, int reps, int choice[2], float *ms, int nms);
+/* Number of strided-pass candidates {y, x} and the ones in use (either
+ * pointer may be NULL); ihg_set_fft_candidates picks them by hand (an index
+ * out of range is an error). */
+int ihg_fft_candidates(ihg_ctx *c, int count[2], int current[2]);
+int ihg_set_fft_candidates(ihg_ctx *c, int y, int x);
+
 /* Switch the GW precision form (ihg_params.gw_fp32) for later calls. */
 int ihg_set_gw_fp32(ihg_ctx *c, int on);
 

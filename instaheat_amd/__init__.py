@@ -129,7 +129,8 @@ IHG_SYMBOLS = [
     "ihg_rk4_stage", "ihg_rk4_final", "ihg_dp853_stage", "ihg_dp853_finish",
     "ihg_dp853_tableau", "ihg_aux", "ihg_prof_enable", "ihg_prof_reset",
     "ihg_prof_read", "ihg_prof_nfam", "ihg_prof_family", "ihg_features",
-    "ihg_set_gw_fp32", "ihg_create_ensemble", "ihg_ensemble",
+    "ihg_set_gw_fp32", "ihg_autotune", "ihg_fft_candidates",
+    "ihg_set_fft_candidates", "ihg_create_ensemble", "ihg_ensemble",
     "ihg_mk_rhs_ensemble", "ihg_graph_begin", "ihg_graph_end",
     "ihg_graph_join", "ihg_graph_launch", "ihg_graph_destroy",
     "ihg_mlp_nparams", "ihg_mlp_create", "ihg_mlp_destroy",
@@ -197,6 +198,9 @@ def lib():
         "ihg_prof_family": (ctypes.c_char_p, [i]),
         "ihg_features": (i, [_vp, _vp, _dptr]),
         "ihg_set_gw_fp32": (i, [_vp, i]),
+        "ihg_autotune": (i, [_vp, i, ctypes.POINTER(i), _fptr, i]),
+        "ihg_fft_candidates": (i, [_vp, ctypes.POINTER(i), ctypes.POINTER(i)]),
+        "ihg_set_fft_candidates": (i, [_vp, i, i]),
         "ihg_create_ensemble": (i, [ctypes.POINTER(IhgParams), i,
                                     ctypes.POINTER(_vp)]),
         "ihg_ensemble": (i, [_vp, ctypes.POINTER(ctypes.c_long)]),

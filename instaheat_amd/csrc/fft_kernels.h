@@ -43,18 +43,22 @@ struct LineSet {
 // loads straight from HBM into registers and the last stage stores straight
 // from registers; only the transposes between stages go through LDS.
 // ---------------------------------------------------------------------------
-template <int L, int DIR, class C = double2>
-__global__ __launch_bounds__(FFT_WG) void k_c2c_strided(
+// The line plan P and the work-group size WG are template parameters so
+// that an autotuner can pick them per line length (B = WG / (L / P::E)
+// lines per work-group).
+template <int L, int DIR, class C = double2, class P = Plan<L>,
+          int WG = FFT_WG>
+__global__ __launch_bounds__(WG) void k_c2c_strided(
     const C *in, C *out, LineSet ls, long batch, const C *__restrict__ tw,
     long rr)
 {
-    using TL = Tile<L>;
-    __shared__ C lds[TL::B * TL::PITCH];
+    constexpr int B = WG / (L / P::E), PITCH = Lds<L>::PITCH;
+    __shared__ C lds[B * PITCH];
     const int tid = threadIdx.x;
     in += blockIdx.y * batch + blockIdx.z * rr;
     out += blockIdx.y * batch + blockIdx.z * rr;
-    const int b = tid % TL::B, j = tid / TL::B;
-    const long line = (long)blockIdx.x * TL::B + b;
+    const int b = tid % B, j = tid / B;
+    const long line = (long)blockIdx.x * B + b;
     const bool act = line < ls.nlines;
     const long base = ls.base(act ? line : 0);
     const C *src = in + base;
@@ -67,7 +71,54 @@ __global__ __launch_bounds__(FFT_WG) void k_c2c_strided(
     auto wr = [&](int, int n, C v) {
         if (act) dst[n * st] = v;
     };
-    fft_line<L, DIR, true, true>(lds + b * TL::PITCH, j, tw, rd, wr);
+    fft_line<L, DIR, true, true, P>(lds + b * PITCH, j, tw, rd, wr);
+}
+
+// Candidate (plan, work-group size) pairs of the strided pass per line
+// length; entry 0 is the default.  The autotuner times them on the
+// context's own arrays and keeps the fastest.
+template <class P_, int WG_> struct StridedCand {
+    using P = P_;
+    static constexpr int WG = WG_;
+};
+template <class... Cs> struct CandList {
+    static constexpr int n = sizeof...(Cs);
+};
+template <int L> struct StridedCands {
+    using type = CandList<StridedCand<Plan<L>, 256>>;
+};
+template <> struct StridedCands<16> {
+    using type = CandList<StridedCand<Plan<16>, 256>, StridedCand<PlanT<4, 4, 4>, 256>,
+                          StridedCand<Plan<16>, 128>>;
+};
+template <> struct StridedCands<32> {
+    using type = CandList<StridedCand<Plan<32>, 256>, StridedCand<PlanT<16, 16, 2>, 256>,
+                          StridedCand<PlanT<4, 4, 4, 2>, 256>, StridedCand<Plan<32>, 128>>;
+};
+template <> struct StridedCands<64> {
+    using type = CandList<StridedCand<Plan<64>, 256>, StridedCand<PlanT<16, 16, 4>, 256>,
+                          StridedCand<PlanT<4, 4, 4, 4>, 256>, StridedCand<Plan<64>, 128>>;
+};
+template <> struct StridedCands<128> {
+    using type = CandList<StridedCand<Plan<128>, 256>, StridedCand<PlanT<8, 8, 8, 2>, 256>,
+                          StridedCand<PlanT<4, 4, 4, 4, 2>, 256>, StridedCand<Plan<128>, 128>>;
+};
+template <> struct StridedCands<256> {
+    using type = CandList<StridedCand<Plan<256>, 256>, StridedCand<PlanT<8, 8, 8, 4>, 512>,
+                          StridedCand<PlanT<4, 4, 4, 4, 4>, 512>>;
+};
+
+// f(StridedCand<...>{}) for candidate `id` of a list (the default for an id
+// out of range)
+template <class F, class C0, class... Cs>
+auto with_cand(int id, CandList<C0, Cs...>, F &&f)
+{
+    if constexpr (sizeof...(Cs) == 0) {
+        return f(C0{});
+    } else {
+        if (id <= 0) return f(C0{});
+        return with_cand(id - 1, CandList<Cs...>{}, f);
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -470,6 +470,40 @@ extern "C" int ihg_mk_rhs(ihg_ctx *c, double t, double *f, double *result,
     return 0;
 }
 
+extern "C" int ihg_autotune(ihg_ctx *c, int reps, int choice[2],
+                            float *ms, int nms)
+{
+    if (!c) return set_error("NULL context");
+    if (reps < 1) reps = 1;
+    float tmp[16] = {0};
+    IHG_TRY(autotune(c, reps, choice, tmp));
+    for (int i = 0; ms && i < nms && i < 16; ++i) ms[i] = tmp[i];
+    return 0;
+}
+
+extern "C" int ihg_fft_candidates(ihg_ctx *c, int count[2], int current[2])
+{
+    if (!c) return set_error("NULL context");
+    int n[2];
+    ihg::candidate_counts(c, n);
+    if (count) count[0] = n[0], count[1] = n[1];
+    if (current) current[0] = c->cand_y, current[1] = c->cand_x;
+    return 0;
+}
+
+extern "C" int ihg_set_fft_candidates(ihg_ctx *c, int y, int x)
+{
+    if (!c) return set_error("NULL context");
+    int n[2];
+    ihg::candidate_counts(c, n);
+    if (y < 0 || y >= n[0] || x < 0 || x >= n[1])
+        return set_error("FFT candidate out of range (y %d of %d, x %d of %d)",
+                         y, n[0], x, n[1]);
+    c->cand_y = y;
+    c->cand_x = x;
+    return 0;
+}
+
 extern "C" int ihg_set_gw_fp32(ihg_ctx *c, int on)
 {
     if (!c) return set_error("NULL context");

@@ -24,20 +24,26 @@ static LineSet x_lines(const ihg_ctx *c)
 template <int L, class C>
 static int launch_c2c(ihg_ctx *c, const C *in, C *out, LineSet ls, int dir,
                       int batch, long cbatch, int fam, const C *tw, int nr,
-                      long rr)
+                      long rr, int cand)
 {
-    using TL = Tile<L>;
-    dim3 grid((unsigned)((ls.nlines + TL::B - 1) / TL::B), batch, nr);
     // algorithmic bytes: every complex value read once and written once
     const double bytes = 2.0 * sizeof(C) * c->M * batch * nr;
-    return timed(c, fam, bytes, [&]() {
-        if (dir < 0)
-            hipLaunchKernelGGL((k_c2c_strided<L, -1, C>), grid, dim3(FFT_WG), 0,
-                               c->stream, in, out, ls, cbatch, tw, rr);
-        else
-            hipLaunchKernelGGL((k_c2c_strided<L, +1, C>), grid, dim3(FFT_WG), 0,
-                               c->stream, in, out, ls, cbatch, tw, rr);
-        return hip_check(hipGetLastError(), "k_c2c_strided");
+    return with_cand(cand, typename StridedCands<L>::type{}, [&](auto cd) {
+        using CD = decltype(cd);
+        using P = typename CD::P;
+        constexpr int B = CD::WG / (L / P::E);
+        dim3 grid((unsigned)((ls.nlines + B - 1) / B), batch, nr);
+        return timed(c, fam, bytes, [&]() {
+            if (dir < 0)
+                hipLaunchKernelGGL((k_c2c_strided<L, -1, C, P, CD::WG>), grid,
+                                   dim3(CD::WG), 0, c->stream, in, out, ls,
+                                   cbatch, tw, rr);
+            else
+                hipLaunchKernelGGL((k_c2c_strided<L, +1, C, P, CD::WG>), grid,
+                                   dim3(CD::WG), 0, c->stream, in, out, ls,
+                                   cbatch, tw, rr);
+            return hip_check(hipGetLastError(), "k_c2c_strided");
+        });
     });
 }
 
@@ -47,7 +53,7 @@ int y_pass(ihg_ctx *c, double2 *data, int dir, int batch, long cbatch,
     return dispatch_len(c->Ny, [&](auto Lc) {
         return launch_c2c<decltype(Lc)::value>(c, data, data, y_lines(c), dir,
                                                batch, cbatch, F_Y_PASS, c->tw,
-                                               nr, rr);
+                                               nr, rr, c->cand_y);
     });
 }
 
@@ -57,7 +63,7 @@ int x_pass(ihg_ctx *c, double2 *data, int dir, int batch, long cbatch,
     return dispatch_len(c->Nx, [&](auto Lc) {
         return launch_c2c<decltype(Lc)::value>(c, data, data, x_lines(c), dir,
                                                batch, cbatch, F_X_PASS, c->tw,
-                                               nr, rr);
+                                               nr, rr, c->cand_x);
     });
 }
 
@@ -67,7 +73,7 @@ int x_pass_f32(ihg_ctx *c, float2 *data, int dir, int batch, long cbatch,
     return dispatch_len(c->Nx, [&](auto Lc) {
         return launch_c2c<decltype(Lc)::value>(c, data, data, x_lines(c), dir,
                                                batch, cbatch, F_X_PASS, c->twf,
-                                               nr, rr);
+                                               nr, rr, c->cand_x);
     });
 }
 
@@ -77,7 +83,7 @@ static int x_pass_oop(ihg_ctx *c, const double2 *in, double2 *out, int dir,
     return dispatch_len(c->Nx, [&](auto Lc) {
         return launch_c2c<decltype(Lc)::value>(c, in, out, x_lines(c), dir,
                                                batch, cbatch, F_X_PASS, c->tw,
-                                               1, 0L);
+                                               1, 0L, c->cand_x);
     });
 }
 
@@ -129,6 +135,70 @@ int fft_c2r(ihg_ctx *c, const double2 *in, double *out, double2 *work,
     IHG_TRY(x_pass_oop(c, in, work, +1, batch, cbatch));
     IHG_TRY(y_pass(c, work, +1, batch, cbatch));
     return z_c2r_pass(c, work, out, batch, cbatch, rbatch);
+}
+
+// Autotuner of the strided passes: every (plan, work-group) candidate of the
+// y and of the x line length is timed on the context's own work arrays, in
+// the shape the RHS uses most (four arrays, all realizations of an ensemble),
+// and the fastest is kept.  Results of different candidates differ only in
+// rounding.
+template <int L>
+static int n_cands()
+{
+    return StridedCands<L>::type::n;
+}
+
+void candidate_counts(const ihg_ctx *c, int n[2])
+{
+    n[0] = dispatch_len(c->Ny, [&](auto Lc) { return n_cands<decltype(Lc)::value>(); });
+    n[1] = dispatch_len(c->Nx, [&](auto Lc) { return n_cands<decltype(Lc)::value>(); });
+}
+
+int autotune(ihg_ctx *c, int reps, int out[2], float ms_out[16])
+{
+    const int nr = c->nreal;
+    const long rr = nr > 1 ? c->ens.w : 0;
+    hipEvent_t a, b;
+    IHG_TRY(hip_check(hipEventCreate(&a), "hipEventCreate"));
+    IHG_TRY(hip_check(hipEventCreate(&b), "hipEventCreate"));
+    const bool prof = c->prof_on;
+    c->prof_on = false;
+    int rc = 0, k = 0;
+    for (int axis = 0; axis < 2 && !rc; ++axis) {
+        const int len = axis ? c->Nx : c->Ny;
+        const int ncand = dispatch_len(len, [&](auto Lc) {
+            return n_cands<decltype(Lc)::value>();
+        });
+        float best = 1e30f;
+        int pick = 0;
+        for (int id = 0; id < ncand && !rc; ++id) {
+            int &slot = axis ? c->cand_x : c->cand_y;
+            slot = id;
+            auto pass = [&]() {
+                return axis ? x_pass(c, c->W, -1, 4, c->M, nr, rr)
+                            : y_pass(c, c->W, -1, 4, c->M, nr, rr);
+            };
+            rc = pass(); // warm-up
+            if (!rc) rc = hip_check(hipEventRecord(a, c->stream), "hipEventRecord");
+            for (int r = 0; r < reps && !rc; ++r) rc = pass();
+            if (!rc) rc = hip_check(hipEventRecord(b, c->stream), "hipEventRecord");
+            if (!rc) rc = hip_check(hipEventSynchronize(b), "hipEventSynchronize");
+            float ms = 0.0f;
+            if (!rc) rc = hip_check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+            if (ms_out && k < 16) ms_out[k] = ms / reps;
+            ++k;
+            if (!rc && ms < best) {
+                best = ms;
+                pick = id;
+            }
+        }
+        (axis ? c->cand_x : c->cand_y) = pick;
+        if (out) out[axis] = pick;
+    }
+    c->prof_on = prof;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return rc;
 }
 
 } // namespace ihg

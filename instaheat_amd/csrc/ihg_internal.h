@@ -108,6 +108,7 @@ struct ihg_ctx {
     long obs_len;
     double *field;     // array mk_summary reads (the reference's `field`)
     int nreal;         // realizations of an ensemble context (1 otherwise)
+    int cand_y = 0, cand_x = 0; // strided-pass candidates (autotuner)
     ihg::Ens ens;      // their strides
     unsigned long calls_rhs;
     int have_obs;
@@ -157,6 +158,8 @@ enum {
 int rhs_core(ihg_ctx *c, double *f, double *result, int output, int kept,
              int nr = 1);
 int features(ihg_ctx *c, const double *f, double out[3]);
+void candidate_counts(const ihg_ctx *c, int n[2]);
+int autotune(ihg_ctx *c, int reps, int out[2], float ms_out[16]);
 int rhs_filter(ihg_ctx *c, double *f, int nr = 1);
 int rhs_gw(ihg_ctx *c, const double *f, double *result, int nr = 1);
 int rhs_output(ihg_ctx *c, const double *f, int kept);

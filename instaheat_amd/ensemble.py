@@ -82,6 +82,17 @@ class Ensemble:
                                       full.ctypes.data_as(ctypes.c_void_p),
                                       full.size))
 
+    def autotune(self, reps=5):
+        """ihg_autotune on this ensemble: the strided-pass candidates in use
+        afterwards {y, x} and every candidate's time (ms)."""
+        choice = (ctypes.c_int * 2)()
+        ms = np.zeros(16, dtype=np.float32)
+        ih._check(ih.lib().ihg_autotune(self.ctx, reps, choice,
+                                        ms.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                        16))
+        self.tuned = (choice[0], choice[1])
+        return self.tuned, ms
+
     def observables(self, r):
         obs = np.zeros(self.obs_len)
         ih._check(ih.host_lib().ih_ens_observables(self.h, r,
@@ -101,7 +112,7 @@ def run(ensembles, nsteps, dt, use_graph=True):
 
 
 def run_mixed(nsteps=1000, dt=0.005, use_graph=True, seed=1, sizes=MIXED,
-              bins=32, keep_states=False):
+              bins=32, keep_states=False, autotune=False):
     """The mixed-size ensemble; returns per size the GW spectrum mean and
     variance over its realizations and the timings."""
     rng = np.random.default_rng(seed)
@@ -113,6 +124,10 @@ def run_mixed(nsteps=1000, dt=0.005, use_graph=True, seed=1, sizes=MIXED,
         e.amplitudes = amps
         seeds += nreal
         ens.append(e)
+    tuned = {}
+    if autotune:
+        for e in ens:
+            tuned[e.n], _ = e.autotune()
     ih.lib().ihg_sync(ens[0].ctx)
     calls0 = sum(e.calls_rhs for e in ens)
     t0 = time.perf_counter()
@@ -121,7 +136,7 @@ def run_mixed(nsteps=1000, dt=0.005, use_graph=True, seed=1, sizes=MIXED,
     out = {"wall_s": wall, "steps": nsteps, "dt": dt, "graph": bool(use_graph),
            "rhs_evals": int(sum(e.calls_rhs for e in ens) - calls0),
            "realization_steps_per_s": sum(e.nreal for e in ens) * nsteps / wall,
-           "sizes": {}}
+           "autotuned": {str(k): v for k, v in tuned.items()}, "sizes": {}}
     for e in ens:
         gw = e.gw_spectra()
         out["sizes"][e.n] = {"nreal": e.nreal, "t": e.t,

@@ -142,3 +142,83 @@ def test_mixed_ensemble_short(E):
         assert np.all(np.isfinite(a["sizes"][n]["gw_var"]))
         assert a["sizes"][n]["t"] == pytest.approx(0.025)
     assert a["rhs_evals"] == 4 * 5 * 6
+
+
+def _candidates(ih, ctx):
+    cnt, cur = (ctypes.c_int * 2)(), (ctypes.c_int * 2)()
+    ih._check(ih.lib().ihg_fft_candidates(ctx, cnt, cur))
+    return list(cnt), list(cur)
+
+
+@pytest.mark.parametrize("shape", [(16, 16, 16), (64, 32, 16), (8, 128, 32),
+                                   (256, 16, 4)])
+def test_every_fft_candidate_matches_oracle(ih, oracle_mod, shape):
+    """Each (plan, work-group) candidate of the strided y and x passes, set by
+    hand: the filtered RHS against the oracle, every block within 1e-12."""
+    from tests.states import blocks, rel_l2
+    y = make_state(shape, seed=11, gw=True, a=1.2)
+    o = oracle_mod.Oracle(*shape, bounds=BOX_CONFIG, mass=4.0, gw=True, bins=20)
+    fo = y.copy()
+    ro = o.mk_rhs(fo, filter=1, output=0, field=fo)
+    B = blocks(shape)
+    g = ih.Gpu(shape, bounds=BOX_CONFIG, mass=4.0, gw=True, bins=20)
+    (ny, nx), cur = _candidates(ih, g.ctx)
+    assert cur == [0, 0]
+    assert ny >= 1 and nx >= 1
+    for cy in range(ny):
+        for cx in range(nx):
+            ih._check(ih.lib().ihg_set_fft_candidates(g.ctx, cy, cx))
+            f = g.to_device(y)
+            r = g.alloc(g.Ntot)
+            g.mk_rhs(f, r, flags=1)
+            got = r.download()
+            err = {k: rel_l2(got[B[k]], ro[B[k]]) for k in B}
+            assert max(err.values()) < 1e-12, (cy, cx, err)
+            assert rel_l2(f.download(), fo) < 1e-12, (cy, cx)
+    with pytest.raises(RuntimeError):
+        ih._check(ih.lib().ihg_set_fft_candidates(g.ctx, ny, 0))
+    with pytest.raises(RuntimeError):
+        ih._check(ih.lib().ihg_set_fft_candidates(g.ctx, 0, -1))
+
+
+def test_autotuned_ensemble_rhs_matches_oracle(ih, E, oracle_mod):
+    """ihg_autotune on a 32^3 x 4 ensemble picks valid candidates, reports a
+    time for each, and the ensemble RHS with its choice still matches the
+    oracle for every realization."""
+    from tests.states import blocks, rel_l2
+    n, nreal = 32, 4
+    e = E.Ensemble(n, nreal, bins=16)
+    e.init([1, 2, 3, 4], [1e-5, 2e-5, 3e-5, 1e-4])
+    counts, _ = _candidates(ih, e.ctx)
+    (cy, cx), ms = e.autotune(reps=3)
+    assert 0 <= cy < counts[0] and 0 <= cx < counts[1]
+    assert _candidates(ih, e.ctx)[1] == [cy, cx]
+    nt = counts[0] + counts[1]
+    assert np.all(ms[:nt] > 0) and np.all(ms[nt:] == 0)
+    states = e.state()
+    L = ih.lib()
+    stride = e.stride
+    nn = nreal * stride
+    fd, rd = ctypes.c_void_p(), ctypes.c_void_p()
+    ih._check(L.ihg_malloc(e.ctx, ctypes.byref(fd), nn))
+    ih._check(L.ihg_malloc(e.ctx, ctypes.byref(rd), nn))
+    full = np.zeros((nreal, stride))
+    full[:, :e.Ntot] = states
+    ih._check(L.ihg_upload(e.ctx, fd, full.ctypes.data_as(ctypes.c_void_p), nn))
+    ih._check(L.ihg_mk_rhs_ensemble(e.ctx, 0.0, fd, rd, 1))
+    got = np.empty(nn)
+    ih._check(L.ihg_download(e.ctx, got.ctypes.data_as(ctypes.c_void_p), rd, nn))
+    got = got.reshape(nreal, stride)
+    shape = (n, n, n)
+    Lb = 2 * np.pi / 0.1
+    o = oracle_mod.Oracle(*shape, bounds=((0.0, Lb),) * 3, mass=1.0, gw=True,
+                          filt=True, bins=16)
+    B = blocks(shape)
+    for r in range(nreal):
+        fo = states[r].copy()
+        ro = o.mk_rhs(fo, filter=1, output=0, field=fo)
+        for k in B:
+            assert rel_l2(got[r, :e.Ntot][B[k]], ro[B[k]]) < 1e-12, (r, k)
+    L.ihg_free(e.ctx, fd)
+    L.ihg_free(e.ctx, rd)
+    e.close()
