@@ -51,6 +51,10 @@ def parse():
     ap.add_argument("--ensemble", action="store_true",
                     help="only run the mixed-size ensemble (48 x 32^3 + 16 x "
                          "64^3 + 4 x 128^3, RK4 dt 0.005) graph vs eager")
+    ap.add_argument("--long-run", type=float, default=0.0, metavar="TF",
+                    help="only run both forms from t = 0 to TF at the grid "
+                         "(RHS evaluations, rejections, wall time and the "
+                         "end-state difference tuned vs plain)")
     ap.add_argument("--copy-probe", type=float, default=0.0, metavar="GB",
                     help="only time a plain copy kernel of GB gigabytes each "
                          "way (the HBM ceiling and the counter calibration)")
@@ -135,16 +139,62 @@ def timed_form(ih, sim, snap, dt0, prop, steps, profile, forced=None, tag=""):
                 last_obs=saves[-1][2] if saves else None)
 
 
-def run_gpu(args, wl):
-    import instaheat_amd as ih
-    from instaheat_amd.proposer import MlpProposer
+def make_sim(ih, wl, skip, tf=1e9):
     sim = ih.Sim(wl["shape"], bounds=wl["bounds"], mass=wl["mass"], gw=wl["gw"],
                  filt=wl["filt"], bins=wl["bins"], method=ih.IH_DOPRI853,
-                 ti=0.0, tf=1e9, dt=1e-7, rtol=1e-10, atol=1e-14,
-                 hubble_frac=1e-2, skip=args.skip)
+                 ti=0.0, tf=tf, dt=1e-7, rtol=1e-10, atol=1e-14,
+                 hubble_frac=1e-2, skip=skip)
     sim.init_gaussian(seed=wl["seed"], amplitude=wl["amplitude"],
                       phi0=wl["phi0"], dphi0=wl["dphi0"], k_cut=wl["kcut"])
     sim.mk_initial_psi()
+    return sim
+
+
+def long_run(args, wl, tf):
+    """Both forms from the same initial state to tf: RHS evaluations,
+    accepted / rejected steps, integration wall time, and how far the tuned
+    form's end state and GW spectrum are from the plain form's."""
+    import instaheat_amd as ih
+    from instaheat_amd.proposer import MlpProposer
+    prop = MlpProposer.load()
+    forms, ends, gws = {}, {}, {}
+    n = wl["shape"][0]
+    live = slice(0, 4 * n ** 3)
+    for form in ("plain", "tuned"):
+        sim = make_sim(ih, wl, args.skip, tf)
+        saves = []
+        r = sim.run_dopri853(proposer=prop if form == "tuned" else None,
+                             saves=saves)
+        calls = int(ih.lib().ihg_calls_rhs(sim.ctx))
+        forms[form] = {"t_end": sim.t, "steps_ok": r["n_ok"], "steps_bad": r["n_bad"],
+                       "rhs_evals": calls,
+                       "integration_s": round(r["integration_seconds"], 3),
+                       "sim_time_per_wall_s": round(sim.t / r["integration_seconds"], 4)}
+        print(f"{form}: {forms[form]}", file=sys.stderr, flush=True)
+        y = sim.field()
+        ends[form] = y
+        gws[form] = ih.unpack_obs(saves[-1][2], wl["bins"])["gw"]
+        forms[form]["a_end"] = float(y[-1])
+        sim.close()
+    p, t = forms["plain"], forms["tuned"]
+    yp, yt = ends["plain"][live], ends["tuned"][live]
+    g0, g1 = gws["plain"], gws["tuned"]
+    ok = g0 > 1e-6 * g0.max()
+    return {"long_run": f"{n}^3 FP64 DOPRI853 rtol 1e-10 atol 1e-14, t = 0 to {tf:g}",
+            "plain": p, "tuned": t,
+            "rhs_evals_tuned_over_plain": round(t["rhs_evals"] / p["rhs_evals"], 4),
+            "speed_tuned_over_plain": round(t["sim_time_per_wall_s"] /
+                                            p["sim_time_per_wall_s"], 4),
+            "end_state_rel_l2_tuned_vs_plain": float(np.linalg.norm(yt - yp) /
+                                                     np.linalg.norm(yp)),
+            "gw_spectrum_rel_err_max_tuned_vs_plain": float(
+                np.max(np.abs(g1[ok] - g0[ok]) / g0[ok]))}
+
+
+def run_gpu(args, wl):
+    import instaheat_amd as ih
+    from instaheat_amd.proposer import MlpProposer
+    sim = make_sim(ih, wl, args.skip)
     y0 = sim.field() if not args.no_cpu_baseline else None
     # warm-up: W accepted steps (the loop takes max_steps + 1 steps)
     sim.set_max_steps(max(args.warmup - 1, 0))
@@ -286,6 +336,9 @@ def main():
         print(json.dumps(ensemble_bench(args.steps)))
         return
     wl = workload(args.grid)
+    if args.long_run > 0:
+        print(json.dumps(long_run(args, wl, args.long_run)))
+        return
     gpu = run_gpu(args, wl)
     tuned, plain = gpu["forms"]["tuned"], gpu["forms"]["plain"]
 

@@ -19,6 +19,7 @@
 #define IHG_H
 
 #include <stddef.h>
+#include <stdint.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -167,6 +168,17 @@ int ihg_graph_end(ihg_ctx *c, ihg_graph **g);
 int ihg_graph_join(ihg_graph *const *parts, int n, ihg_graph **g);
 int ihg_graph_launch(ihg_graph *g, ihg_ctx *c);
 int ihg_graph_destroy(ihg_graph *g);
+
+/* Seeded Gaussian initial data, generated on the device (the reference
+ * generates its initial data on the host, src/setup.c:771-790): field (Ntot
+ * doubles) = phi0 + the c2r of a half spectrum of complex Gaussians of
+ * variance amplitude^2 / (2 a0^3 omega_k), omega_k = sqrt(k^2 + m^2), for
+ * 0 < |k| <= k_cut; dphi0; psi, dpsi and h zero; a0 last.  Each mode's
+ * numbers depend on (seed, mode index) only (splitmix64 counters, Box-Muller
+ * with a uniform phase).  Uses the context's spectral work arrays. */
+int ihg_init_gaussian(ihg_ctx *c, double *field, uint64_t seed,
+                      double amplitude, double phi0, double dphi0, double a0,
+                      double k_cut);
 
 /* Autotuner of the strided y and x passes (no counterpart in the
  * reference): every candidate line plan and work-group size for the y and

@@ -89,6 +89,12 @@ int ih_init_gaussian(ih_sim *s, uint64_t seed, double amplitude, double phi0,
                      double dphi0, double k_cut);
 /* mk_initial_psi (src/setup.c:825-903) on the current field */
 int ih_mk_initial_psi(ih_sim *s);
+/* initial data from a .dat file (read_initial_data, src/io.c:801-820, and
+ * init_from_dat, src/setup.c:802-813): N lines "i j k phi dphi psi dpsi";
+ * with_psi = 0 is IC_FROM_DAT_FILE_WITHOUT_PSI (psi from mk_initial_psi) */
+int ih_read_dat(ih_sim *s, const char *path, int with_psi);
+/* the current phi, dphi, psi, dpsi in that format (no reference writer) */
+int ih_write_dat(ih_sim *s, const char *path);
 
 /* run_rk4 (src/rk4.c:27-132); returns the number of steps or < 0 */
 long ih_run_rk4(ih_sim *s, ih_save_fn save, void *user);

@@ -129,7 +129,7 @@ IHG_SYMBOLS = [
     "ihg_rk4_stage", "ihg_rk4_final", "ihg_dp853_stage", "ihg_dp853_finish",
     "ihg_dp853_tableau", "ihg_aux", "ihg_prof_enable", "ihg_prof_reset",
     "ihg_prof_read", "ihg_prof_nfam", "ihg_prof_family", "ihg_features",
-    "ihg_set_gw_fp32", "ihg_autotune", "ihg_fft_candidates",
+    "ihg_set_gw_fp32", "ihg_init_gaussian", "ihg_autotune", "ihg_fft_candidates",
     "ihg_set_fft_candidates", "ihg_create_ensemble", "ihg_ensemble",
     "ihg_mk_rhs_ensemble", "ihg_graph_begin", "ihg_graph_end",
     "ihg_graph_join", "ihg_graph_launch", "ihg_graph_destroy",
@@ -141,7 +141,8 @@ HOST_SYMBOLS = [
     "ih_config_defaults", "ih_create", "ih_destroy", "ih_gpu", "ih_cfg",
     "ih_field", "ih_dfield", "ih_flags", "ih_time", "ih_set_time", "ih_set_dt",
     "ih_set_max_steps", "ih_set_tf",
-    "ih_mk_rhs", "ih_init_gaussian", "ih_mk_initial_psi", "ih_run_rk4",
+    "ih_mk_rhs", "ih_init_gaussian", "ih_mk_initial_psi", "ih_read_dat",
+    "ih_write_dat", "ih_run_rk4",
     "ih_run_dopri853", "ih_success", "ih_error_norm", "ih_proposer_features",
     "ih_mlp_propose", "ih_ens_create", "ih_ens_destroy", "ih_ens_gpu",
     "ih_ens_state", "ih_ens_stride", "ih_ens_nreal", "ih_ens_time",
@@ -199,6 +200,7 @@ def lib():
         "ihg_features": (i, [_vp, _vp, _dptr]),
         "ihg_set_gw_fp32": (i, [_vp, i]),
         "ihg_autotune": (i, [_vp, i, ctypes.POINTER(i), _fptr, i]),
+        "ihg_init_gaussian": (i, [_vp, _vp, ctypes.c_uint64, d, d, d, d, d]),
         "ihg_fft_candidates": (i, [_vp, ctypes.POINTER(i), ctypes.POINTER(i)]),
         "ihg_set_fft_candidates": (i, [_vp, i, i]),
         "ihg_create_ensemble": (i, [ctypes.POINTER(IhgParams), i,
@@ -255,6 +257,8 @@ def host_lib():
         "ih_flags": (_vp, [_vp]),
         "ih_mk_rhs": (i, [_vp, d, _vp, _vp]),
         "ih_init_gaussian": (i, [_vp, ctypes.c_uint64, d, d, d, d]),
+        "ih_read_dat": (i, [_vp, ctypes.c_char_p, i]),
+        "ih_write_dat": (i, [_vp, ctypes.c_char_p]),
         "ih_mk_initial_psi": (i, [_vp]),
         "ih_run_rk4": (ctypes.c_long, [_vp, SAVE_FN, _vp]),
         "ih_run_dopri853": (ctypes.c_long, [_vp, ctypes.POINTER(IhDpOpts),
@@ -600,6 +604,14 @@ class Sim:
 
     def mk_initial_psi(self):
         _check(host_lib().ih_mk_initial_psi(self.sim))
+
+    def read_dat(self, path, with_psi=True):
+        """initial data from a .dat file (IC_FROM_DAT_FILE_WITH_PSI, or
+        WITHOUT_PSI with with_psi=False)"""
+        _check(host_lib().ih_read_dat(self.sim, os.fsencode(path), int(with_psi)))
+
+    def write_dat(self, path):
+        _check(host_lib().ih_write_dat(self.sim, os.fsencode(path)))
 
     def run_rk4(self, saves=None):
         def cb(user, t, a, obs, n):

@@ -32,7 +32,7 @@ C_FLAGS = ["-std=c99", "-O2", "-fPIC", "-Wall", "-Wextra",
            "-I", os.path.join(ROOT, "include")]
 
 HIP_SOURCES = ["ihg_api.hip", "ihg_fft.hip", "ihg_rhs.hip", "ihg_obs.hip",
-               "ihg_step.hip", "ihg_mlp.hip", "ihg_graph.hip"]
+               "ihg_step.hip", "ihg_mlp.hip", "ihg_graph.hip", "ihg_ic.hip"]
 HOST_SOURCES = ["instaheat.c", "compat.c"]
 
 

@@ -35,6 +35,27 @@ struct LineSet {
     }
 };
 
+// Tile index of this work-group along blockIdx.x.  The dispatcher hands
+// work-groups to the eight XCDs in turn (work-group w to XCD w mod 8, w the
+// linear work-group id; w and w + 8 land on one XCD whatever the other grid
+// dimensions are), and every XCD has its own L2.  Strided tiles that are
+// neighbours share a 128-byte line at their edges (the half-spectrum rows of
+// Nz/2 + 1 complex values start off line boundaries), so with SWZ each XCD
+// gets one contiguous run of tiles: XCD x takes tiles
+// x*q + min(x, r) + [0, q + (x < r)), q = gx / 8, r = gx mod 8, in order.
+// The map is a bijection of [0, gx).
+template <bool SWZ>
+__device__ __forceinline__ unsigned tile_x()
+{
+    if constexpr (!SWZ) {
+        return blockIdx.x;
+    } else {
+        const unsigned gx = gridDim.x, bx = blockIdx.x;
+        const unsigned q = gx / 8, r = gx % 8, x = bx % 8;
+        return x * q + (x < r ? x : r) + bx / 8;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // strided complex-to-complex pass (y or x), in place or out of place,
 // batched over blockIdx.y (arrays `batch` complex values apart).
@@ -47,7 +68,7 @@ struct LineSet {
 // that an autotuner can pick them per line length (B = WG / (L / P::E)
 // lines per work-group).
 template <int L, int DIR, class C = double2, class P = Plan<L>,
-          int WG = FFT_WG>
+          int WG = FFT_WG, bool SWZ = true>
 __global__ __launch_bounds__(WG) void k_c2c_strided(
     const C *in, C *out, LineSet ls, long batch, const C *__restrict__ tw,
     long rr)
@@ -58,7 +79,7 @@ __global__ __launch_bounds__(WG) void k_c2c_strided(
     in += blockIdx.y * batch + blockIdx.z * rr;
     out += blockIdx.y * batch + blockIdx.z * rr;
     const int b = tid % B, j = tid / B;
-    const long line = (long)blockIdx.x * B + b;
+    const long line = (long)tile_x<SWZ>() * B + b;
     const bool act = line < ls.nlines;
     const long base = ls.base(act ? line : 0);
     const C *src = in + base;
@@ -77,35 +98,40 @@ __global__ __launch_bounds__(WG) void k_c2c_strided(
 // Candidate (plan, work-group size) pairs of the strided pass per line
 // length; entry 0 is the default.  The autotuner times them on the
 // context's own arrays and keeps the fastest.
-template <class P_, int WG_> struct StridedCand {
+template <class P_, int WG_, bool SWZ_ = true> struct StridedCand {
     using P = P_;
     static constexpr int WG = WG_;
+    static constexpr bool SWZ = SWZ_;
 };
 template <class... Cs> struct CandList {
     static constexpr int n = sizeof...(Cs);
 };
 template <int L> struct StridedCands {
-    using type = CandList<StridedCand<Plan<L>, 256>>;
+    using type = CandList<StridedCand<Plan<L>, 256>, StridedCand<Plan<L>, 256, false>>;
 };
 template <> struct StridedCands<16> {
-    using type = CandList<StridedCand<Plan<16>, 256>, StridedCand<PlanT<4, 4, 4>, 256>,
-                          StridedCand<Plan<16>, 128>>;
+    using type = CandList<StridedCand<Plan<16>, 256>, StridedCand<Plan<16>, 256, false>,
+                          StridedCand<PlanT<4, 4, 4>, 256>, StridedCand<Plan<16>, 128>>;
 };
 template <> struct StridedCands<32> {
-    using type = CandList<StridedCand<Plan<32>, 256>, StridedCand<PlanT<16, 16, 2>, 256>,
+    using type = CandList<StridedCand<Plan<32>, 256>, StridedCand<Plan<32>, 256, false>,
+                          StridedCand<PlanT<16, 16, 2>, 256>,
                           StridedCand<PlanT<4, 4, 4, 2>, 256>, StridedCand<Plan<32>, 128>>;
 };
 template <> struct StridedCands<64> {
-    using type = CandList<StridedCand<Plan<64>, 256>, StridedCand<PlanT<16, 16, 4>, 256>,
+    using type = CandList<StridedCand<Plan<64>, 256>, StridedCand<Plan<64>, 256, false>,
+                          StridedCand<PlanT<16, 16, 4>, 256>,
                           StridedCand<PlanT<4, 4, 4, 4>, 256>, StridedCand<Plan<64>, 128>>;
 };
 template <> struct StridedCands<128> {
-    using type = CandList<StridedCand<Plan<128>, 256>, StridedCand<PlanT<8, 8, 8, 2>, 256>,
+    using type = CandList<StridedCand<Plan<128>, 256>, StridedCand<Plan<128>, 256, false>,
+                          StridedCand<PlanT<8, 8, 8, 2>, 256>,
                           StridedCand<PlanT<4, 4, 4, 4, 2>, 256>, StridedCand<Plan<128>, 128>>;
 };
 template <> struct StridedCands<256> {
-    using type = CandList<StridedCand<Plan<256>, 256>, StridedCand<PlanT<8, 8, 8, 4>, 512>,
-                          StridedCand<PlanT<4, 4, 4, 4, 4>, 512>>;
+    using type = CandList<StridedCand<Plan<256>, 256>, StridedCand<Plan<256>, 256, false>,
+                          StridedCand<PlanT<8, 8, 8, 4>, 512>,
+                          StridedCand<PlanT<4, 4, 4, 4, 4>, 512, false>>;
 };
 
 // f(StridedCand<...>{}) for candidate `id` of a list (the default for an id
@@ -135,7 +161,7 @@ auto with_cand(int id, CandList<C0, Cs...>, F &&f)
 // transforms run.
 // ---------------------------------------------------------------------------
 template <int L, int NQ, bool FWD, class Fac, class P = Plan<L>,
-          int WG = FFT_WG>
+          int WG = FFT_WG, bool SWZ = true>
 __global__ __launch_bounds__(WG) void k_x_fanout(
     const double2 *in, double2 *out, long qstride, LineSet ls, long batch,
     const double2 *__restrict__ tw, Fac fac, long rr)
@@ -146,7 +172,7 @@ __global__ __launch_bounds__(WG) void k_x_fanout(
     in += blockIdx.y * batch + blockIdx.z * rr;
     out += blockIdx.y * batch + blockIdx.z * rr;
     const int b = tid % B, j = tid / B;
-    const long line = (long)blockIdx.x * B + b;
+    const long line = (long)tile_x<SWZ>() * B + b;
     const bool act = line < ls.nlines;
     const long base = ls.base(act ? line : 0);
     const double2 *src = in + base;

@@ -35,11 +35,11 @@ static int launch_c2c(ihg_ctx *c, const C *in, C *out, LineSet ls, int dir,
         dim3 grid((unsigned)((ls.nlines + B - 1) / B), batch, nr);
         return timed(c, fam, bytes, [&]() {
             if (dir < 0)
-                hipLaunchKernelGGL((k_c2c_strided<L, -1, C, P, CD::WG>), grid,
+                hipLaunchKernelGGL((k_c2c_strided<L, -1, C, P, CD::WG, CD::SWZ>), grid,
                                    dim3(CD::WG), 0, c->stream, in, out, ls,
                                    cbatch, tw, rr);
             else
-                hipLaunchKernelGGL((k_c2c_strided<L, +1, C, P, CD::WG>), grid,
+                hipLaunchKernelGGL((k_c2c_strided<L, +1, C, P, CD::WG, CD::SWZ>), grid,
                                    dim3(CD::WG), 0, c->stream, in, out, ls,
                                    cbatch, tw, rr);
             return hip_check(hipGetLastError(), "k_c2c_strided");

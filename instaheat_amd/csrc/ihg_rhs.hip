@@ -422,7 +422,7 @@ __global__ __launch_bounds__(WG) void k_y_tt_update_h(
     constexpr int E = P::E, T = L / E, B = WG / T, PITCH = Lds<L>::PITCH;
     __shared__ C lds[B * PITCH];
     const int tid = threadIdx.x, b = tid % B, j = tid / B;
-    const long line = (long)blockIdx.x * B + b;
+    const long line = (long)tile_x<true>() * B + b;
     const bool act = line < ls.nlines;
     const long lc = act ? line : 0;
     const long base = ls.base(lc);

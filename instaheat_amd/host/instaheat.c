@@ -10,6 +10,7 @@
 
 #include <float.h>
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
@@ -182,88 +183,69 @@ static int prepare_and_save(ih_sim *s, ih_save_fn save, void *user)
 /* initial data                                                             */
 /* ------------------------------------------------------------------------ */
 
-static uint64_t splitmix64(uint64_t x)
-{
-    x += 0x9E3779B97F4A7C15ull;
-    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
-    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
-    return x ^ (x >> 31);
-}
-
-/* uniform in (0, 1] from 53 random bits */
-static double unit_open0(uint64_t r)
-{
-    return ((double)(r >> 11) + 1.0) * (1.0 / 9007199254740992.0);
-}
-
-/* Gaussian initial data (the build's seeded recipe) into the device state
- * `field` (Ntot doubles) of a grid; `work` holds at least 2M doubles. */
-static int gaussian_into(ihg_ctx *g, const ih_config *cfg, size_t N, size_t M,
-                         size_t Ntot, double *field, double *work,
-                         uint64_t seed, double amplitude, double phi0,
-                         double dphi0, double k_cut)
-{
-    const ihg_params *p = &cfg->grid;
-    const int Nx = p->nx, Ny = p->ny, Nz = p->nz, Mz = Nz / 2 + 1;
-    const double TWOPI = 6.283185307179586476925286766559;
-    const double kx1 = TWOPI / (p->xb - p->xa), ky1 = TWOPI / (p->yb - p->ya),
-                 kz1 = TWOPI / (p->zb - p->za);
-    const double a0 = cfg->a_initial;
-    const double m2 = p->mass * p->mass;
-    double *spec = calloc(2 * M, sizeof(double));
-    double *real = calloc(Ntot, sizeof(double));
-    if (!spec || !real) {
-        free(spec);
-        free(real);
-        return IHG_ENOMEM;
-    }
-    for (int i = 0; i < Nx; ++i) {
-        const double kx = kx1 * (i > Nx / 2 ? i - Nx : i);
-        for (int j = 0; j < Ny; ++j) {
-            const double ky = ky1 * (j > Ny / 2 ? j - Ny : j);
-            for (int k = 0; k < Mz; ++k) {
-                const double kz = kz1 * k;
-                const double k2 = kx * kx + ky * ky + kz * kz;
-                if (k2 == 0.0 || k2 > k_cut * k_cut) continue;
-                const size_t id = ((size_t)i * Ny + j) * Mz + k;
-                const uint64_t base = splitmix64(seed ^ 0x5DEECE66Dull) + 2 * id;
-                const double u1 = unit_open0(splitmix64(base));
-                const double u2 = unit_open0(splitmix64(base + 1)) - 0x1p-53;
-                const double omega = sqrt(k2 + m2);
-                const double sigma =
-                    amplitude * sqrt(1.0 / (2.0 * a0 * a0 * a0 * omega));
-                const double r = sqrt(-2.0 * log(u1)) * sigma / sqrt(2.0);
-                spec[2 * id] = r * cos(TWOPI * u2);
-                spec[2 * id + 1] = r * sin(TWOPI * u2);
-            }
-        }
-    }
-    int rc = ihg_zero(g, field, Ntot);
-    if (!rc) rc = ihg_upload(g, work, spec, 2 * M);
-    if (!rc) rc = ihg_fft_c2r(g, work, field);
-    if (!rc) rc = ihg_download(g, real, field, N);
-    if (!rc) {
-        for (size_t i = 0; i < N; ++i) {
-            real[i] += phi0;
-            real[N + i] = dphi0;
-        }
-        real[Ntot - 1] = a0;
-        rc = ihg_upload(g, field, real, Ntot);
-    }
-    if (!rc) rc = ihg_zero(g, work, 2 * M);
-    free(spec);
-    free(real);
-    return rc;
-}
-
 int ih_init_gaussian(ih_sim *s, uint64_t seed, double amplitude, double phi0,
                      double dphi0, double k_cut)
 {
     int rc = ihg_zero(s->g, s->dfield, s->Ntot);
     if (!rc)
-        rc = gaussian_into(s->g, &s->cfg, s->N, s->M, s->Ntot, s->field,
-                           s->k_tmp, seed, amplitude, phi0, dphi0, k_cut);
+        rc = ihg_init_gaussian(s->g, s->field, seed, amplitude, phi0, dphi0,
+                               s->cfg.a_initial, k_cut);
     s->t = s->cfg.ti;
+    return rc;
+}
+
+/* read_initial_data + init_from_dat (src/io.c:801-820, src/setup.c:802-813):
+ * N lines "i j k phi dphi psi dpsi" in the order of the flat index, a =
+ * A_INITIAL, and with !with_psi (IC_FROM_DAT_FILE_WITHOUT_PSI) psi and dpsi
+ * from mk_initial_psi.  Every other element of the state is zero.  A line
+ * with fewer than seven numbers, or a file with fewer than N lines, is an
+ * error here (the reference only catches a line with no number at all). */
+int ih_read_dat(ih_sim *s, const char *path, int with_psi)
+{
+    FILE *fp = fopen(path, "r");
+    if (!fp) return IHG_EINVAL;
+    const size_t N = s->N;
+    double *h = calloc(s->Ntot, sizeof(double));
+    int rc = h ? 0 : IHG_ENOMEM;
+    for (size_t i = 0; !rc && i < N; ++i) {
+        int ii, jj, kk;
+        if (fscanf(fp, " %d %d %d %lf %lf %lf %lf", &ii, &jj, &kk, &h[i],
+                   &h[i + N], &h[i + 2 * N], &h[i + 3 * N]) != 7)
+            rc = IHG_EINVAL;
+    }
+    fclose(fp);
+    if (!rc) {
+        h[s->Ntot - 1] = s->cfg.a_initial;
+        rc = ihg_upload(s->g, s->field, h, s->Ntot);
+    }
+    if (!rc) rc = ihg_zero(s->g, s->dfield, s->Ntot);
+    free(h);
+    s->t = s->cfg.ti;
+    if (!rc && !with_psi) rc = ih_mk_initial_psi(s);
+    return rc;
+}
+
+/* the current phi, dphi, psi, dpsi in the format ih_read_dat reads, with
+ * (i, j, k) the grid indices of each line and 17 significant digits (the
+ * values read back are the same doubles) */
+int ih_write_dat(ih_sim *s, const char *path)
+{
+    const size_t N = s->N;
+    double *h = malloc(4 * N * sizeof(double));
+    if (!h) return IHG_ENOMEM;
+    int rc = ihg_download(s->g, h, s->field, 4 * N);
+    FILE *fp = rc ? NULL : fopen(path, "w");
+    if (!rc && !fp) rc = IHG_EINVAL;
+    const int Ny = s->cfg.grid.ny, Nz = s->cfg.grid.nz;
+    for (size_t i = 0; !rc && i < N; ++i) {
+        const int x = (int)(i / ((size_t)Ny * Nz)), y = (int)(i / Nz % Ny),
+                  z = (int)(i % Nz);
+        if (fprintf(fp, "%d %d %d %.17g %.17g %.17g %.17g\n", x, y, z, h[i],
+                    h[i + N], h[i + 2 * N], h[i + 3 * N]) < 0)
+            rc = IHG_EINVAL;
+    }
+    if (fp && fclose(fp) != 0 && !rc) rc = IHG_EINVAL;
+    free(h);
     return rc;
 }
 
@@ -641,8 +623,8 @@ int ih_ens_init(ih_ens *e, const uint64_t *seeds, const double *amplitudes,
     int rc = 0;
     for (int r = 0; !rc && r < e->nreal; ++r) {
         double *f = e->y + (size_t)r * e->stride;
-        rc = gaussian_into(e->g, &e->cfg, e->N, e->M, e->Ntot, f, e->tmp,
-                           seeds[r], amplitudes[r], phi0, dphi0, k_cut);
+        rc = ihg_init_gaussian(e->g, f, seeds[r], amplitudes[r], phi0, dphi0,
+                               e->cfg.a_initial, k_cut);
         /* mk_initial_psi (src/setup.c:825-903) of this realization */
         if (!rc) rc = ihg_set_field(e->g, f);
         if (!rc) rc = ihg_mk_initial_psi(e->g, e->cfg.ti, f, e->k[0]);
