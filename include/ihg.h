@@ -129,6 +129,21 @@ int ihg_dp853_finish(ihg_ctx *c, double dt, const double *y,
 int ihg_dp853_tableau(double *a, double *c, double *b, double *bhh,
                       double *er);
 
+/* DOPRI853 dense output (the continuous extension whose coefficients sit
+ * unused in src/dopri853_constants.c:23-25, 98-173).  After an accepted step
+ * y -> y_new of size dt, k[0..15]: k[0] the step's FSAL derivative, k[j]
+ * stage j+1 (j = 1..11), k[12] = f(t + dt, y_new), k[13..15] the three extra
+ * stages.  ihg_dp853_dense_stage (s = 14..16): out = y + dt * sum_j
+ * a_{s,j} k[j], the input of extra stage s at t + c_s dt.
+ * ihg_dp853_dense: out = the interpolant at t + theta dt, theta in [0, 1]
+ * (k[1..4] are not read).  ihg_dp853_dense_tableau: c_extra[3],
+ * a_extra[3][16], d[4][16], row-major. */
+int ihg_dp853_dense_stage(ihg_ctx *c, int s, double dt, const double *y,
+                          const double *const *k, double *out);
+int ihg_dp853_dense(ihg_ctx *c, double theta, double dt, const double *y,
+                    const double *y_new, const double *const *k, double *out);
+int ihg_dp853_dense_tableau(double *c_extra, double *a_extra, double *d);
+
 /* Live timing of the kernel families (the runtime_* monitors of
  * src/main_template.h:292-311, per kernel): when enabled, every launch is
  * bracketed by HIP events on the context's stream.  ihg_prof_read returns,

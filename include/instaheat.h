@@ -132,7 +132,16 @@ typedef struct ih_dp_opts {
     long log_cap;
     ih_dt_proposer proposer;  /* may be NULL */
     void *proposer_user;
+    /* time-based output by dense output: after every accepted step from t
+     * to t + dt, save() is called for each t_out[i] in (t, t + dt] (the
+     * step-based outputs every `skip` steps are kept); t_out increasing */
+    const double *t_out;
+    long n_out;
 } ih_dp_opts;
+
+/* the state of the latest dense output (device, Ntot doubles); valid
+ * inside the save() call of a time-based output */
+double *ih_dense_state(ih_sim *s);
 
 /* run_dopri853 (src/dopri853.c:112-158) */
 long ih_run_dopri853(ih_sim *s, const ih_dp_opts *opts, ih_save_fn save,

@@ -99,7 +99,8 @@ PROPOSER_FN = ctypes.CFUNCTYPE(ctypes.c_double, _vp,
 class IhDpOpts(ctypes.Structure):
     _fields_ = [("forced_dt", _dptr), ("n_forced", ctypes.c_long),
                 ("log", ctypes.POINTER(IhDpRecord)), ("log_cap", ctypes.c_long),
-                ("proposer", PROPOSER_FN), ("proposer_user", _vp)]
+                ("proposer", PROPOSER_FN), ("proposer_user", _vp),
+                ("t_out", _dptr), ("n_out", ctypes.c_long)]
 
 
 class IhMlpProposer(ctypes.Structure):
@@ -127,7 +128,8 @@ IHG_SYMBOLS = [
     "ihg_sync", "ihg_fft_r2c", "ihg_fft_c2r", "ihg_set_field", "ihg_mk_rhs",
     "ihg_means", "ihg_observables", "ihg_calls_rhs", "ihg_mk_initial_psi",
     "ihg_rk4_stage", "ihg_rk4_final", "ihg_dp853_stage", "ihg_dp853_finish",
-    "ihg_dp853_tableau", "ihg_aux", "ihg_prof_enable", "ihg_prof_reset",
+    "ihg_dp853_tableau", "ihg_dp853_dense_stage", "ihg_dp853_dense",
+    "ihg_dp853_dense_tableau", "ihg_aux", "ihg_prof_enable", "ihg_prof_reset",
     "ihg_prof_read", "ihg_prof_nfam", "ihg_prof_family", "ihg_features",
     "ihg_set_gw_fp32", "ihg_init_gaussian", "ihg_autotune", "ihg_fft_candidates",
     "ihg_set_fft_candidates", "ihg_create_ensemble", "ihg_ensemble",
@@ -143,7 +145,7 @@ HOST_SYMBOLS = [
     "ih_set_max_steps", "ih_set_tf",
     "ih_mk_rhs", "ih_init_gaussian", "ih_mk_initial_psi", "ih_read_dat",
     "ih_write_dat", "ih_run_rk4",
-    "ih_run_dopri853", "ih_success", "ih_error_norm", "ih_proposer_features",
+    "ih_run_dopri853", "ih_dense_state", "ih_success", "ih_error_norm", "ih_proposer_features",
     "ih_mlp_propose", "ih_ens_create", "ih_ens_destroy", "ih_ens_gpu",
     "ih_ens_state", "ih_ens_stride", "ih_ens_nreal", "ih_ens_time",
     "ih_ens_calls_rhs", "ih_ens_init", "ih_ens_capture_step",
@@ -191,6 +193,9 @@ def lib():
         "ihg_dp853_finish": (i, [_vp, d, _vp, ctypes.POINTER(_vp), _vp, d, d,
                                  _dptr]),
         "ihg_dp853_tableau": (i, [_dptr] * 5),
+        "ihg_dp853_dense_stage": (i, [_vp, i, d, _vp, _vp, _vp]),
+        "ihg_dp853_dense": (i, [_vp, d, d, _vp, _vp, _vp, _vp]),
+        "ihg_dp853_dense_tableau": (i, [_dptr] * 3),
         "ihg_aux": (i, [_vp, i, _dptr]),
         "ihg_prof_enable": (i, [_vp, i]),
         "ihg_prof_reset": (i, [_vp]),
@@ -246,6 +251,7 @@ def host_lib():
         "ih_create": (i, [ctypes.POINTER(IhConfig), ctypes.POINTER(_vp)]),
         "ih_destroy": (None, [_vp]),
         "ih_gpu": (_vp, [_vp]),
+        "ih_dense_state": (_vp, [_vp]),
         "ih_field": (_vp, [_vp]),
         "ih_dfield": (_vp, [_vp]),
         "ih_time": (d, [_vp]),
@@ -605,6 +611,13 @@ class Sim:
     def mk_initial_psi(self):
         _check(host_lib().ih_mk_initial_psi(self.sim))
 
+    def dense_state(self):
+        """the latest dense-output state (host copy)"""
+        out = np.empty(self.Ntot)
+        _check(lib().ihg_download(self.ctx, out.ctypes.data_as(ctypes.c_void_p),
+                                  host_lib().ih_dense_state(self.sim), self.Ntot))
+        return out
+
     def read_dat(self, path, with_psi=True):
         """initial data from a .dat file (IC_FROM_DAT_FILE_WITH_PSI, or
         WITHOUT_PSI with with_psi=False)"""
@@ -623,14 +636,20 @@ class Sim:
             _check(n)
         return n
 
-    def run_dopri853(self, forced=None, saves=None, log_cap=0, proposer=None):
+    def run_dopri853(self, forced=None, saves=None, log_cap=0, proposer=None,
+                     t_out=None, on_save=None):
         """run_dopri853; `proposer` is a Python callable
         (history records, n, dt_classic) -> dt, or an object with a
         ``c_proposer()`` method returning (C function, user pointer), such as
-        instaheat_amd.proposer.MlpProposer."""
+        instaheat_amd.proposer.MlpProposer.  `t_out`: increasing times of
+        dense (time-based) outputs; `on_save(t)` runs inside every save()
+        call (where dense_state() is the state of a time-based output)."""
         fz = None
         if forced is not None:
             fz = np.ascontiguousarray(forced, dtype=np.float64)
+        to = None
+        if t_out is not None:
+            to = np.ascontiguousarray(t_out, dtype=np.float64)
         log = (IhDpRecord * log_cap)() if log_cap else None
         user = None
         if proposer is None:
@@ -642,12 +661,16 @@ class Sim:
         opts = IhDpOpts(fz.ctypes.data_as(_dptr) if fz is not None else None,
                         len(fz) if fz is not None else 0,
                         ctypes.cast(log, ctypes.POINTER(IhDpRecord)) if log else None,
-                        log_cap, prop, user)
+                        log_cap, prop, user,
+                        to.ctypes.data_as(_dptr) if to is not None else None,
+                        len(to) if to is not None else 0)
         st = IhDpStats()
 
         def cb(user, t, a, obs, n):
             if saves is not None:
                 saves.append((t, a, np.ctypeslib.as_array(obs, (n,)).copy()))
+            if on_save is not None:
+                on_save(t)
         fn = SAVE_FN(cb)
         n = host_lib().ih_run_dopri853(self.sim, ctypes.byref(opts), fn, None,
                                        ctypes.byref(st))

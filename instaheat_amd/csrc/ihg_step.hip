@@ -79,6 +79,42 @@ __global__ __launch_bounds__(RED_WG) void k_dp_stage(
     }
 }
 
+// dense output: out = y + beta (y_new - y) + dt * sum_j w_j k_j, the
+// interpolant of an accepted step at one theta (weights from the host)
+__global__ __launch_bounds__(RED_WG) void k_dp_dense(
+    double *__restrict__ out, const double *__restrict__ y,
+    const double *__restrict__ y_new, double beta, StageIn s, double dt,
+    long nlive, long Ntot)
+{
+    const long npairs = nlive / 2;
+    const long stride = (long)gridDim.x * RED_WG;
+    const double2 *y2 = reinterpret_cast<const double2 *>(y);
+    const double2 *n2 = reinterpret_cast<const double2 *>(y_new);
+    double2 *o2 = reinterpret_cast<double2 *>(out);
+    for (long i = (long)blockIdx.x * RED_WG + threadIdx.x; i < npairs;
+         i += stride) {
+        double ax = 0.0, ay = 0.0;
+#pragma unroll
+        for (int j = 0; j < 12; ++j) {
+            if (j < s.nk) {
+                const double2 kj = reinterpret_cast<const double2 *>(s.k[j])[i];
+                ax += s.a[j] * kj.x;
+                ay += s.a[j] * kj.y;
+            }
+        }
+        const double2 yy = y2[i], yn = n2[i];
+        o2[i] = make_double2(yy.x + beta * (yn.x - yy.x) + dt * ax,
+                             yy.y + beta * (yn.y - yy.y) + dt * ay);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const long i = Ntot - 1;
+        double acc = 0.0;
+        for (int j = 0; j < s.nk; ++j)
+            acc += s.a[j] * s.k[j][i];
+        out[i] = y[i] + beta * (y_new[i] - y[i]) + dt * acc;
+    }
+}
+
 // RK4 stage: out = y + dt * k / 2 (half) or y + dt * k, over the pairs of
 // [0, nlive) and the element `tail` (the scale factor; -1: none)
 __global__ __launch_bounds__(RED_WG) void k_rk4_stage(
@@ -276,6 +312,98 @@ extern "C" int ihg_dp853_stage(ihg_ctx *c, int s, double dt, const double *y,
                            c->Ntot);
         return hip_check(hipGetLastError(), "k_dp_stage");
     });
+}
+
+extern "C" int ihg_dp853_dense_stage(ihg_ctx *c, int s, double dt,
+                                     const double *y, const double *const *k,
+                                     double *out)
+{
+    if (!c || !k) return set_error("NULL argument");
+    if (s < 14 || s > 16) return set_error("dense stage %d out of range 14..16", s);
+    IHG_TRY(check_ptr(y, "y"));
+    IHG_TRY(check_ptr(out, "out"));
+    StageIn in{};
+    int n = 0;
+    for (int j = 0; j < s - 1; ++j) {
+        const double a = DP_A_EXTRA[s - 14][j];
+        if (a == 0.0) continue;
+        IHG_TRY(check_ptr(k[j], "k[j]"));
+        in.k[n] = k[j];
+        in.a[n] = a;
+        ++n;
+    }
+    in.nk = n;
+    const long nl = live_len(c);
+    return timed(c, F_DP_DENSE, (n + 2.0) * 8.0 * nl, [&]() {
+        hipLaunchKernelGGL(k_dp_stage, dim3(stream_blocks(nl / 2)),
+                           dim3(RED_WG), 0, c->stream, out, y, in, dt, nl,
+                           c->Ntot);
+        return hip_check(hipGetLastError(), "k_dp_stage");
+    });
+}
+
+// The interpolant of DOP853's dense output at theta in [0, 1]:
+//   y(theta) = y + theta (F0 + (1-theta) (F1 + theta (F2 + (1-theta) (F3 +
+//              theta (F4 + (1-theta) (F5 + theta F6))))))
+// with F0 = y_new - y, F1 = dt k0 - F0, F2 = 2 F0 - dt (k12 + k0) and
+// F3..F6 = dt sum_j DP_D[r][j] k_j, written as y + beta (y_new - y) +
+// dt sum_j w_j k_j with the weights of the nested form's products
+// p_m = theta^ceil((m+1)/2) (1-theta)^floor((m+1)/2), m = 0..6.
+extern "C" int ihg_dp853_dense(ihg_ctx *c, double theta, double dt,
+                               const double *y, const double *y_new,
+                               const double *const *k, double *out)
+{
+    if (!c || !k) return set_error("NULL argument");
+    if (!(theta >= 0.0 && theta <= 1.0))
+        return set_error("dense output at theta = %g outside [0, 1]", theta);
+    IHG_TRY(check_ptr(y, "y"));
+    IHG_TRY(check_ptr(y_new, "y_new"));
+    IHG_TRY(check_ptr(out, "out"));
+    const double u = 1.0 - theta;
+    double p[7];
+    p[0] = theta;
+    for (int m = 1; m < 7; ++m)
+        p[m] = p[m - 1] * ((m & 1) ? u : theta);
+    double w[16] = {0.0};
+    w[0] = p[1] - p[2];
+    w[12] = -p[2];
+    for (int r = 0; r < 4; ++r)
+        for (int j = 0; j < 16; ++j)
+            w[j] += p[3 + r] * DP_D[r][j];
+    const double beta = p[0] - p[1] + 2.0 * p[2];
+    StageIn in{};
+    int n = 0;
+    for (int j = 0; j < 16; ++j) {
+        if (w[j] == 0.0) continue;
+        if (n == 12) return set_error("dense output: more than 12 stages");
+        IHG_TRY(check_ptr(k[j], "k[j]"));
+        in.k[n] = k[j];
+        in.a[n] = w[j];
+        ++n;
+    }
+    in.nk = n;
+    const long nl = live_len(c);
+    return timed(c, F_DP_DENSE, (n + 3.0) * 8.0 * nl, [&]() {
+        hipLaunchKernelGGL(k_dp_dense, dim3(stream_blocks(nl / 2)),
+                           dim3(RED_WG), 0, c->stream, out, y, y_new, beta, in,
+                           dt, nl, c->Ntot);
+        return hip_check(hipGetLastError(), "k_dp_dense");
+    });
+}
+
+extern "C" int ihg_dp853_dense_tableau(double *c_extra, double *a_extra,
+                                       double *d)
+{
+    if (!c_extra || !a_extra || !d) return set_error("NULL argument");
+    for (int i = 0; i < 3; ++i) {
+        c_extra[i] = DP_C_EXTRA[i];
+        for (int j = 0; j < 16; ++j)
+            a_extra[i * 16 + j] = DP_A_EXTRA[i][j];
+    }
+    for (int r = 0; r < 4; ++r)
+        for (int j = 0; j < 16; ++j)
+            d[r * 16 + j] = DP_D[r][j];
+    return 0;
 }
 
 extern "C" int ihg_dp853_finish(ihg_ctx *c, double dt, const double *y,

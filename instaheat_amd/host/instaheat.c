@@ -31,6 +31,7 @@ struct ih_sim {
     double *obs;         /* host copy of the observables */
     size_t obs_len;
     double dp_c[13];     /* DOPRI853 nodes c_s */
+    double dp_c_extra[3]; /* nodes of the dense output's extra stages */
     /* the last IH_HISTORY DOPRI853 attempts, oldest first (the step
        proposer's inputs); kept across runs, cleared by ih_set_time */
     ih_dp_record hist[IH_HISTORY];
@@ -122,6 +123,10 @@ int ih_create(const ih_config *cfg, ih_sim **out)
         double a[13 * 12], b[12], bhh[3], er[12];
         rc = ihg_dp853_tableau(a, s->dp_c, b, bhh, er);
     }
+    if (!rc) {
+        double ax[3 * 16], d[4 * 16];
+        rc = ihg_dp853_dense_tableau(s->dp_c_extra, ax, d);
+    }
     if (rc) {
         ih_destroy(s);
         return rc;
@@ -165,6 +170,57 @@ static int do_save(ih_sim *s, ih_save_fn save, void *user, double t)
     save(user, t, a, s->obs, s->obs_len);
     return 0;
 }
+
+/* DOPRI853 dense output of one accepted step from t0 to t0 + h, called
+ * after the swap: field = y_new, dfield = f(t0 + h, y_new), field_new = y,
+ * dfield_new = the step's FSAL derivative, stage[1..11] = stages 2..12.
+ * For every t_out in (t0, t0 + h]: the three extra stages once (into the
+ * arrays of stages 2..4, which the interpolant does not read), the
+ * interpolated state into stage[0], an output mk_rhs on it with the
+ * summaries reading it, and save(t_out). */
+static int dense_saves(ih_sim *s, const ih_dp_opts *o, long *next, double t0,
+                       double h, ih_save_fn save, void *user)
+{
+    long i = *next;
+    while (i < o->n_out && o->t_out[i] <= t0) ++i;
+    *next = i;
+    if (i >= o->n_out || o->t_out[i] > t0 + h) return 0;
+    const double *k[16];
+    k[0] = s->dfield_new;
+    for (int j = 1; j < 12; ++j) k[j] = s->stage[j];
+    k[12] = s->dfield;
+    k[13] = s->stage[1];
+    k[14] = s->stage[2];
+    k[15] = s->stage[3];
+    int rc = 0;
+    for (int st = 14; st <= 16 && !rc; ++st) {
+        rc = ihg_dp853_dense_stage(s->g, st, h, s->field_new, k, s->k_tmp);
+        if (!rc)
+            rc = ih_mk_rhs(s, t0 + s->dp_c_extra[st - 14] * h, s->k_tmp,
+                           s->stage[st - 13]);
+    }
+    double *yd = s->stage[0];
+    for (; !rc && i < o->n_out && o->t_out[i] <= t0 + h; ++i) {
+        double theta = (o->t_out[i] - t0) / h;
+        if (theta > 1.0) theta = 1.0;
+        rc = ihg_dp853_dense(s->g, theta, h, s->field_new, s->field, k, yd);
+        if (rc) break;
+        ihg_set_field(s->g, yd);
+        s->evo_flags.output = 1;
+        rc = ih_mk_rhs(s, o->t_out[i], yd, s->k_tmp);
+        s->evo_flags.output = 0;
+        ihg_set_field(s->g, s->field);
+        if (rc || !save) continue;
+        double a;
+        rc = ihg_observables(s->g, s->obs, s->obs_len);
+        if (!rc) rc = ihg_download(s->g, &a, yd + s->Ntot - 1, 1);
+        if (!rc) save(user, o->t_out[i], a, s->obs, s->obs_len);
+    }
+    *next = i;
+    return rc;
+}
+
+double *ih_dense_state(ih_sim *s) { return s->stage[0]; }
 
 /* prepare_and_save_timeslice, src/toolbox.c:96-104 */
 static int prepare_and_save(ih_sim *s, ih_save_fn save, void *user)
@@ -373,7 +429,7 @@ long ih_run_dopri853(ih_sim *s, const ih_dp_opts *opts, ih_save_fn save,
     if (!s->stage[11]) return IHG_EINVAL; /* created for RK4 only */
     ih_ctrl ctl = {1.0e-4, c->dt, 0};
     double dt = c->dt, last_save_t = s->t;
-    long n_stp, n_ok = 0, n_bad = 0, attempts = 0;
+    long n_stp, n_ok = 0, n_bad = 0, attempts = 0, next_out = 0;
     int rc;
     const int filter = c->grid.enable_filter ? 1 : 0;
 
@@ -467,7 +523,12 @@ long ih_run_dopri853(ih_sim *s, const ih_dp_opts *opts, ih_save_fn save,
         s->dfield = s->dfield_new;
         s->dfield_new = tmp;
         ihg_set_field(s->g, s->field);
+        const double t_old = s->t;
         s->t += h;
+        if (opts->t_out) {
+            rc = dense_saves(s, opts, &next_out, t_old, h, save, user);
+            if (rc) return rc;
+        }
         if (h == dt)
             ++n_ok;
         else
