@@ -108,30 +108,38 @@ template <class... Cs> struct CandList {
 };
 template <int L> struct StridedCands {
     using type = CandList<StridedCand<Plan<L>, 256>, StridedCand<Plan<L>, 256, false>>;
+    static constexpr int x_default = 0;
 };
 template <> struct StridedCands<16> {
     using type = CandList<StridedCand<Plan<16>, 256>, StridedCand<Plan<16>, 256, false>,
                           StridedCand<PlanT<4, 4, 4>, 256>, StridedCand<Plan<16>, 128>>;
+    static constexpr int x_default = 0;
 };
 template <> struct StridedCands<32> {
     using type = CandList<StridedCand<Plan<32>, 256>, StridedCand<Plan<32>, 256, false>,
                           StridedCand<PlanT<16, 16, 2>, 256>,
                           StridedCand<PlanT<4, 4, 4, 2>, 256>, StridedCand<Plan<32>, 128>>;
+    static constexpr int x_default = 0;
 };
 template <> struct StridedCands<64> {
     using type = CandList<StridedCand<Plan<64>, 256>, StridedCand<Plan<64>, 256, false>,
                           StridedCand<PlanT<16, 16, 4>, 256>,
                           StridedCand<PlanT<4, 4, 4, 4>, 256>, StridedCand<Plan<64>, 128>>;
+    static constexpr int x_default = 0;
 };
 template <> struct StridedCands<128> {
     using type = CandList<StridedCand<Plan<128>, 256>, StridedCand<Plan<128>, 256, false>,
                           StridedCand<PlanT<8, 8, 8, 2>, 256>,
                           StridedCand<PlanT<4, 4, 4, 4, 2>, 256>, StridedCand<Plan<128>, 128>>;
+    static constexpr int x_default = 0;
 };
+// 256-point x lines (points a whole y-z plane apart) run fastest with 32
+// lines per work-group (512 B contiguous per point); y lines with 16.
 template <> struct StridedCands<256> {
     using type = CandList<StridedCand<Plan<256>, 256>, StridedCand<Plan<256>, 256, false>,
-                          StridedCand<PlanT<8, 8, 8, 4>, 512>,
+                          StridedCand<Plan<256>, 512>, StridedCand<PlanT<8, 8, 8, 4>, 512>,
                           StridedCand<PlanT<4, 4, 4, 4, 4>, 512, false>>;
+    static constexpr int x_default = 2;
 };
 
 // f(StridedCand<...>{}) for candidate `id` of a list (the default for an id

@@ -210,6 +210,7 @@ static int create_ctx(const ihg_params *p, int nreal, ihg_ctx **out)
     c->Ny = p->ny;
     c->Nz = p->nz;
     c->Mz = p->nz / 2 + 1;
+    c->cand_x = ihg::default_candidate_x(c->Nx);
     c->N = (long)c->Nx * c->Ny * c->Nz;
     c->M = (long)c->Nx * c->Ny * c->Mz;
     c->Next = 2 * c->M;

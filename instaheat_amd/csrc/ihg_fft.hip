@@ -148,6 +148,13 @@ static int n_cands()
     return StridedCands<L>::type::n;
 }
 
+int default_candidate_x(int len)
+{
+    return dispatch_len(len, [&](auto Lc) {
+        return StridedCands<decltype(Lc)::value>::x_default;
+    });
+}
+
 void candidate_counts(const ihg_ctx *c, int n[2])
 {
     n[0] = dispatch_len(c->Ny, [&](auto Lc) { return n_cands<decltype(Lc)::value>(); });

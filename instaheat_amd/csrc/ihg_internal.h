@@ -158,6 +158,7 @@ enum {
 int rhs_core(ihg_ctx *c, double *f, double *result, int output, int kept,
              int nr = 1);
 int features(ihg_ctx *c, const double *f, double out[3]);
+int default_candidate_x(int len);
 void candidate_counts(const ihg_ctx *c, int n[2]);
 int autotune(ihg_ctx *c, int reps, int out[2], float ms_out[16]);
 int rhs_filter(ihg_ctx *c, double *f, int nr = 1);
