@@ -194,6 +194,41 @@ __device__ __forceinline__ void dp_finish_point(const FinishIn &f, long i,
     e5 += t * t;
 }
 
+// the same for the elements 2i, 2i+1 (their error terms summed in that
+// order)
+__device__ __forceinline__ void dp_finish_pair(const FinishIn &f, long i,
+                                               double2 yv, double dt,
+                                               double rtol, double atol,
+                                               double2 &ynew, double &e5,
+                                               double &e3)
+{
+    const double2 *k[12];
+#pragma unroll
+    for (int j = 0; j < 12; ++j)
+        k[j] = reinterpret_cast<const double2 *>(f.k[j]);
+    const double2 k0 = k[0][i], k5 = k[5][i], k6 = k[6][i], k7 = k[7][i],
+                  k8 = k[8][i], k9 = k[9][i], k10 = k[10][i], k11 = k[11][i];
+    auto one = [&](double y0, double a0, double a5, double a6, double a7,
+                   double a8, double a9, double a10, double a11) {
+        const double bs = DP_B[0] * a0 + DP_B[5] * a5 + DP_B[6] * a6 +
+                          DP_B[7] * a7 + DP_B[8] * a8 + DP_B[9] * a9 +
+                          DP_B[10] * a10 + DP_B[11] * a11;
+        const double yn = y0 + dt * bs;
+        const double yerr = bs - DP_BHH[0] * a0 - DP_BHH[1] * a8 - DP_BHH[2] * a11;
+        const double yerr2 = DP_ER[0] * a0 + DP_ER[5] * a5 + DP_ER[6] * a6 +
+                             DP_ER[7] * a7 + DP_ER[8] * a8 + DP_ER[9] * a9 +
+                             DP_ER[10] * a10 + DP_ER[11] * a11;
+        const double sk = atol + rtol * fmax(fabs(y0), fabs(yn));
+        double t = yerr / sk;
+        e3 += t * t;
+        t = yerr2 / sk;
+        e5 += t * t;
+        return yn;
+    };
+    ynew.x = one(yv.x, k0.x, k5.x, k6.x, k7.x, k8.x, k9.x, k10.x, k11.x);
+    ynew.y = one(yv.y, k0.y, k5.y, k6.y, k7.y, k8.y, k9.y, k10.y, k11.y);
+}
+
 // b-weighted update, embedded errors and the two sums of error()
 __global__ __launch_bounds__(RED_WG) void k_dp_finish(
     const double *__restrict__ y, FinishIn f, double *__restrict__ ynew,
@@ -203,11 +238,17 @@ __global__ __launch_bounds__(RED_WG) void k_dp_finish(
     __shared__ double sh[8];
     double e5 = 0.0, e3 = 0.0;
     const long stride = (long)gridDim.x * RED_WG;
-    for (long i = (long)blockIdx.x * RED_WG + threadIdx.x; i < nlive;
+    // pairs of elements (16-byte loads and stores; nlive is even), then the
+    // scale factor a
+    const long npairs = nlive / 2;
+    const double2 *y2 = reinterpret_cast<const double2 *>(y);
+    double2 *n2 = reinterpret_cast<double2 *>(ynew);
+    for (long i = (long)blockIdx.x * RED_WG + threadIdx.x; i < npairs;
          i += stride) {
-        double yn;
-        dp_finish_point(f, i, y[i], dt, rtol, atol, yn, e5, e3);
-        ynew[i] = yn;
+        double2 yn;
+        const double2 yv = y2[i];
+        dp_finish_pair(f, i, yv, dt, rtol, atol, yn, e5, e3);
+        n2[i] = yn;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         double yn;
@@ -422,8 +463,10 @@ extern "C" int ihg_dp853_finish(ihg_ctx *c, double dt, const double *y,
     for (int j = 1; j <= 4; ++j)
         in.k[j] = k[0];
     const long nl = live_len(c);
-    long nb = (nl + RED_WG * 8 - 1) / (RED_WG * 8);
-    if (nb > 2048) nb = 2048;
+    // about four pairs per thread: enough work-groups in flight to keep the
+    // nine input streams at full rate (the block partials stay small)
+    long nb = (nl / 2 + RED_WG * 4 - 1) / (RED_WG * 4);
+    if (nb > 32768) nb = 32768;
     if (nb < 1) nb = 1;
     if (2 * nb + 2 > c->npartials) return set_error("partials buffer too small");
     // y and the 8 weighted stages in, y_new out
