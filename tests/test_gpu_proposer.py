@@ -134,3 +134,30 @@ def test_tuned_form_accuracy_against_oracle(ih, P, oracle_mod):
           "tuned vs plain", rel_l2(ends["tuned"][live], ends["plain"][live]))
     assert err["plain"] < 1e-6
     assert err["tuned"] < 1.5 * err["plain"] + 1e-9
+
+
+@pytest.mark.parametrize("n,gw,tf", [(32, True, 30.0), (64, True, 1.0)])
+def test_tuned_form_matches_oracle_on_its_steps(ih, P, oracle_mod, n, gw, tf):
+    """The tuned form's end state against the FP64 oracle taking the same
+    accepted steps (forced): the implementation error of the tuned run,
+    which must stay within 1e-9 relative."""
+    prop = P.MlpProposer.load()
+    sim = P.make_sim(n, gw, 1e-10, 1e-4, 33, tf)
+    y0 = sim.field()
+    r = sim.run_dopri853(proposer=prop, log_cap=100000)
+    assert sim.t == pytest.approx(tf, rel=1e-14)
+    log = r["log"]
+    dts = log[log[:, 7] > 0.5, 1]
+    assert r["n_bad"] == int((log[:, 7] < 0.5).sum())
+    L = 2 * np.pi / 0.1
+    o = oracle_mod.Oracle(n, n, n, bounds=((0.0, L),) * 3, mass=1.0, gw=gw,
+                          filt=True, bins=16)
+    yo = y0.copy()
+    ro = o.run_dopri853(yo, np.zeros_like(yo), 0.0, tf, 1e-7, rtol=1e-10,
+                        atol=1e-14, hubble_frac=1e-2, skip=10**9, forced=dts)
+    assert ro["t"] == pytest.approx(sim.t, rel=1e-14)
+    err = rel_l2(sim.field(), yo)
+    print(f"{n}^3 tuned form: {len(dts)} accepted + {r['n_bad']} rejected steps, "
+          f"{prop.calls} proposals; end state vs the oracle on the same steps {err:.3e}")
+    assert prop.calls > 0
+    assert err < 1e-9
