@@ -195,7 +195,6 @@ def run_gpu(args, wl):
     import instaheat_amd as ih
     from instaheat_amd.proposer import MlpProposer
     sim = make_sim(ih, wl, args.skip)
-    y0 = sim.field() if not args.no_cpu_baseline else None
     # warm-up: W accepted steps (the loop takes max_steps + 1 steps)
     sim.set_max_steps(max(args.warmup - 1, 0))
     w = sim.run_dopri853()
@@ -221,9 +220,18 @@ def run_gpu(args, wl):
     g32 = ih.unpack_obs(forms["precision"]["last_obs"], wl["bins"])["gw"]
     live = g64 > 1e-6 * g64.max()
     forms["precision"]["gw_rel_err"] = (np.abs(g32 - g64)[live] / g64[live]).tolist()
+    check = None
+    if not args.no_cpu_baseline:
+        # ORACLE_STEPS forced steps of the plain form's first step size from
+        # the warm-up state, for the oracle to take on the host cores
+        sim.restore(snap)
+        y_snap, t_snap = sim.field(), sim.t
+        dts = np.full(ORACLE_STEPS, forms["plain"]["dts"][0][0])
+        sim.run_dopri853(forced=dts)
+        check = dict(y=y_snap, t=t_snap, dts=dts, y_end=sim.field())
     sim.drop(snap)
     sim.close()
-    return dict(forms=forms, y0=y0, warm_t=w["t"])
+    return dict(forms=forms, check=check, warm_t=w["t"])
 
 
 def ensemble_bench(steps):
@@ -295,32 +303,39 @@ def roofline(prof, grid=256):
             "kernels": kernels}
 
 
-def cpu_baseline(args, wl, gpu):
+ORACLE_STEPS = 2
+
+
+def cpu_baseline(args, wl, check, dt_mean):
     """The oracle (C99 + OpenMP restatement of the reference) on the host
-    cores: one plain mk_rhs and one filtered + output mk_rhs at the same
-    256^3 state, scaled to a DOPRI853 step (11 plain RHS + 1 filtered FSAL
-    call; stage arithmetic not counted), then to simulated time per wall
-    second with the GPU run's mean accepted step size."""
+    cores: ORACLE_STEPS forced DOPRI853 steps from the warm-up state (its
+    initial filtered output call, then 11 stages + the FSAL call per step,
+    stage arithmetic included), the same steps the GPU took.  Its seconds per
+    step give simulated time per wall second at the plain form's mean
+    accepted step; its end state is the GPU's parity check at full size."""
     from oracle import oracle as orc
     threads = args.cpu_threads or len(os.sched_getaffinity(0))
     n = wl["shape"][0]
-    y = gpu["y0"]
     o = orc.Oracle(n, n, n, bounds=wl["bounds"], mass=wl["mass"], gw=wl["gw"],
                    filt=wl["filt"], bins=wl["bins"], threads=threads)
-    f = y.copy()
+    y = check["y"].copy()
     t0 = time.perf_counter()
-    o.mk_rhs(f, field=f)
-    t1 = time.perf_counter()
-    o.mk_rhs(f, filter=1, output=1, field=f)
-    t2 = time.perf_counter()
-    step_s = 11 * (t1 - t0) + (t2 - t1)
-    return {"value": round(gpu["dt"] / step_s, 6), "unit": "sim-time/wall-s",
+    o.run_dopri853(y, np.zeros_like(y), check["t"], 1e9, check["dts"][0],
+                   rtol=1e-10, atol=1e-14, hubble_frac=1e-2, skip=10**9,
+                   forced=check["dts"])
+    secs = time.perf_counter() - t0
+    step_s = secs / len(check["dts"])
+    err = float(np.linalg.norm(check["y_end"] - y) / np.linalg.norm(y))
+    return {"value": round(dt_mean / step_s, 6), "unit": "sim-time/wall-s",
             "cores": threads, "kind": "port",
-            "sample": (f"oracle (C99+OpenMP) on {threads} host threads: 1 plain "
-                       f"+ 1 filter/output mk_rhs at {n}^3 GW on "
-                       f"({t1 - t0:.2f} s, {t2 - t1:.2f} s), scaled to 11+1 RHS "
-                       f"per step and the GPU run's mean dt {gpu['dt']:.4g}"),
-            "seconds_per_step": round(step_s, 3)}
+            "sample": (f"oracle (C99+OpenMP) on {threads} host threads: "
+                       f"{len(check['dts'])} forced DOPRI853 steps of dt "
+                       f"{check['dts'][0]:.4g} from the {n}^3 warm-up state at "
+                       f"t = {check['t']:.3f} ({secs:.1f} s, incl. the initial "
+                       f"output call), scaled by the GPU plain form's mean dt "
+                       f"{dt_mean:.4g}"),
+            "seconds_per_step": round(step_s, 3),
+            "gpu_end_state_rel_err_vs_oracle": err}
 
 
 def main():
@@ -383,7 +398,7 @@ def main():
         "roofline": roofline(plain["prof"], args.grid),
     }
     if not args.no_cpu_baseline:
-        rec["cpu_baseline"] = cpu_baseline(args, wl, dict(y0=gpu["y0"], dt=plain["dt"]))
+        rec["cpu_baseline"] = cpu_baseline(args, wl, gpu["check"], plain["dt"])
     else:
         rec["cpu_baseline"] = None
     print(json.dumps(rec))
