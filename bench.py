@@ -18,7 +18,9 @@ A third form, precision (the GW source and spectrum in FP32), replays the
 plain form's steps and reports its GW spectrum error per k bin.
 value = the tuned form's simulated time advanced / wall seconds of its K
 timed steps (the mon.integration span), every array resident in HBM; the
-plain form's figures are reported beside it.
+plain form's figures are reported beside it.  The per-kernel figures of the
+roofline come from one more replay of the plain form's steps under the
+live profiler.
 
 Prints ONE JSON line.  --gpus > 1 is refused: this build runs on one GPU.
 """
@@ -206,8 +208,14 @@ def run_gpu(args, wl):
     for form in ("plain", "tuned"):
         forms[form] = timed_form(ih, sim, snap, w["dt_next"],
                                  prop if form == "tuned" else None,
-                                 args.steps, profile=(form == "plain"), tag=form)
+                                 args.steps, profile=False, tag=form)
     forms["tuned"]["proposals"] = int(prop.calls)
+    # per-kernel figures: the plain form's steps once more under the live
+    # profiler (which keeps every launch on one stream, so each kernel is
+    # timed alone)
+    prof = timed_form(ih, sim, snap, w["dt_next"], None, args.steps,
+                      profile=True, forced=forms["plain"]["dts"],
+                      tag="profile")["prof"]
     # the precision form (FP32 GW source and spectrum) over the plain form's
     # steps, its GW spectrum per k bin against the plain form's
     ih._check(ih.lib().ihg_set_gw_fp32(sim.ctx, 1))
@@ -231,7 +239,7 @@ def run_gpu(args, wl):
         check = dict(y=y_snap, t=t_snap, dts=dts, y_end=sim.field())
     sim.drop(snap)
     sim.close()
-    return dict(forms=forms, check=check, warm_t=w["t"])
+    return dict(forms=forms, prof=prof, check=check, warm_t=w["t"])
 
 
 def ensemble_bench(steps):
@@ -395,7 +403,7 @@ def main():
                 gw_spectrum_rel_err_median=float(np.median(gpu["forms"]["precision"]["gw_rel_err"])),
                 gw_bins_compared=len(gpu["forms"]["precision"]["gw_rel_err"])),
         },
-        "roofline": roofline(plain["prof"], args.grid),
+        "roofline": roofline(gpu["prof"], args.grid),
     }
     if not args.no_cpu_baseline:
         rec["cpu_baseline"] = cpu_baseline(args, wl, gpu["check"], plain["dt"])
