@@ -1,8 +1,8 @@
 /*
  * oracle.c -- CPU restatement (C99 + OpenMP) of InstaHeat's RHS and steppers.
  *
- * TEST INFRASTRUCTURE ONLY (the checker and the timed CPU baseline).  The
- * shipped HIP path never links or calls this file.
+ * TEST INFRASTRUCTURE ONLY (the parity reference of the tests and the timed
+ * CPU baseline).  The shipped HIP path never links or calls this file.
  *
  * Each function cites the reference function it restates (paths relative to
  * the reference repository root).  Loops are written in the reference's

@@ -113,7 +113,8 @@ def test_compat_layer_exports_the_reference_names(ih):
 
 
 def test_c_host_compiles_against_the_headers(tmp_path, ih):
-    """A C99 host written as in INTEGRATION.md compiles and links."""
+    """A C99 host that replaces the reference's run_dopri853 call with the
+    library (the compat layer's names) compiles and links."""
     import subprocess
     src = tmp_path / "host.c"
     src.write_text(r"""
