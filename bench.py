@@ -38,6 +38,9 @@ sys.path.insert(0, HERE)
 METRIC = "sim-time/wall-s at 256³ FP64 DOPRI853 rtol 1e-10: tuned vs plain GPU"
 FFT_FAMILIES = ("z_r2c", "z_c2r", "y_pass", "x_pass", "x_fanout", "z_c2r_rho",
                 "z_r2c_sources", "y_tt_update_h", "x_roundtrip")
+# the passes that do nothing but transform (the others fuse derivatives,
+# rho and p, the GW products, the TT projection or the filter into a pass)
+FFT_PLAIN = ("z_r2c", "z_c2r", "y_pass", "x_pass")
 HBM_PEAK_GBS = 8000.0
 
 
@@ -297,6 +300,8 @@ def roofline(prof, grid=256):
     ach = per_launch / avg_s / 1e9
     fft_ms = sum(prof[k][0] for k in FFT_FAMILIES if k in prof)
     fft_by = sum(prof[k][1] for k in FFT_FAMILIES if k in prof)
+    pl_ms = sum(prof[k][0] for k in FFT_PLAIN if k in prof)
+    pl_by = sum(prof[k][1] for k in FFT_PLAIN if k in prof)
     kernels = {k: {"ms": round(v[0], 3), "launches": v[2],
                    "avg_us": round(1e3 * v[0] / v[2], 2),
                    "GB_per_s": round(v[1] / (v[0] / 1e3) / 1e9, 1)}
@@ -308,6 +313,8 @@ def roofline(prof, grid=256):
             "bytes_per_launch": per_launch, "avg_launch_us": round(avg_s * 1e6, 2),
             "fft_all_passes_GB_per_s": round(fft_by / (fft_ms / 1e3) / 1e9, 1),
             "fft_all_passes_frac": round(fft_by / (fft_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+            "fft_plain_passes_GB_per_s": round(pl_by / (pl_ms / 1e3) / 1e9, 1),
+            "fft_plain_passes_frac": round(pl_by / (pl_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
             "kernels": kernels}
 
 
