@@ -196,12 +196,14 @@ int ihg_init_gaussian(ihg_ctx *c, double *field, uint64_t seed,
                       double k_cut);
 
 /* Autotuner of the strided y and x passes (no counterpart in the
- * reference): every candidate line plan and work-group size for the y and
- * the x line length is timed `reps` times on the context's work arrays (in
- * the RHS's shape: four arrays, every realization of an ensemble) and the
- * fastest is kept for later calls.  choice = {y, x} candidate indices (0 is
- * the default); ms (may be NULL) receives up to nms average times, y
- * candidates first.  Candidates differ in rounding only. */
+ * reference): every candidate line plan, work-group size and tile order for
+ * the y and the x line length is timed (three rounds of `reps` passes on the
+ * context's work arrays, in the RHS's shape: four arrays, every realization
+ * of an ensemble; each candidate's fastest round counts), and a candidate
+ * replaces the default only when it is at least 3 % faster.  choice = {y, x}
+ * candidate indices in use afterwards; ms (may be NULL) receives up to nms
+ * average pass times, y candidates first.  Candidates differ in rounding
+ * only. */
 int ihg_autotune(ihg_ctx *c, int reps, int choice[2], float *ms, int nms);
 /* Number of strided-pass candidates {y, x} and the ones in use (either
  * pointer may be NULL); ihg_set_fft_candidates picks them by hand (an index

@@ -176,30 +176,37 @@ int autotune(ihg_ctx *c, int reps, int out[2], float ms_out[16])
         const int ncand = dispatch_len(len, [&](auto Lc) {
             return n_cands<decltype(Lc)::value>();
         });
-        float best = 1e30f;
-        int pick = 0;
-        for (int id = 0; id < ncand && !rc; ++id) {
-            int &slot = axis ? c->cand_x : c->cand_y;
-            slot = id;
-            auto pass = [&]() {
-                return axis ? x_pass(c, c->W, -1, 4, c->M, nr, rr)
-                            : y_pass(c, c->W, -1, 4, c->M, nr, rr);
-            };
-            rc = pass(); // warm-up
-            if (!rc) rc = hip_check(hipEventRecord(a, c->stream), "hipEventRecord");
-            for (int r = 0; r < reps && !rc; ++r) rc = pass();
-            if (!rc) rc = hip_check(hipEventRecord(b, c->stream), "hipEventRecord");
-            if (!rc) rc = hip_check(hipEventSynchronize(b), "hipEventSynchronize");
-            float ms = 0.0f;
-            if (!rc) rc = hip_check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
-            if (ms_out && k < 16) ms_out[k] = ms / reps;
-            ++k;
-            if (!rc && ms < best) {
-                best = ms;
-                pick = id;
+        const int def = axis ? default_candidate_x(len) : 0;
+        int &slot = axis ? c->cand_x : c->cand_y;
+        // three rounds over all candidates, each candidate's fastest round
+        // kept (drifts of the clock or of the other work hit every candidate)
+        float tmin[16];
+        for (int id = 0; id < 16; ++id)
+            tmin[id] = 1e30f;
+        for (int round = 0; round < 3 && !rc; ++round) {
+            for (int id = 0; id < ncand && id < 16 && !rc; ++id) {
+                slot = id;
+                auto pass = [&]() {
+                    return axis ? x_pass(c, c->W, -1, 4, c->M, nr, rr)
+                                : y_pass(c, c->W, -1, 4, c->M, nr, rr);
+                };
+                rc = pass(); // warm-up
+                if (!rc) rc = hip_check(hipEventRecord(a, c->stream), "hipEventRecord");
+                for (int r = 0; r < reps && !rc; ++r) rc = pass();
+                if (!rc) rc = hip_check(hipEventRecord(b, c->stream), "hipEventRecord");
+                if (!rc) rc = hip_check(hipEventSynchronize(b), "hipEventSynchronize");
+                float ms = 0.0f;
+                if (!rc) rc = hip_check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+                if (!rc && ms < tmin[id]) tmin[id] = ms;
             }
         }
-        (axis ? c->cand_x : c->cand_y) = pick;
+        // the default stays unless a candidate is at least 3 % faster
+        int pick = def;
+        for (int id = 0; id < ncand && id < 16; ++id)
+            if (tmin[id] < 0.97f * tmin[def] && tmin[id] < tmin[pick]) pick = id;
+        for (int id = 0; id < ncand && id < 16; ++id, ++k)
+            if (ms_out && k < 16) ms_out[k] = tmin[id] / reps;
+        slot = pick;
         if (out) out[axis] = pick;
     }
     c->prof_on = prof;
