@@ -39,13 +39,31 @@ static int check_ptr(const void *p, const char *name)
 }
 
 template <class T>
-static int dalloc(T **p, size_t count)
+static int dalloc(T **p, size_t count, hipStream_t stream)
 {
+    // zeroed on the context's stream and finished before returning: the
+    // context's stream does not wait for the legacy null stream, so a
+    // null-stream memset could still be running when the stream's next
+    // upload or kernel writes the array
     *p = nullptr;
     if (count == 0) count = 1;
     hipError_t e = hipMalloc((void **)p, count * sizeof(T));
     if (e != hipSuccess) return hip_check(e, "hipMalloc");
-    return hip_check(hipMemset(*p, 0, count * sizeof(T)), "hipMemset");
+    IHG_TRY(hip_check(hipMemsetAsync(*p, 0, count * sizeof(T), stream),
+                      "hipMemsetAsync"));
+    return hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+}
+
+// host -> device copy at set-up, on the context's stream and finished
+// before returning (a null-stream copy from pageable memory may still be in
+// flight when it returns, unordered with the context's stream)
+static int upload_now(void *dst, const void *src, size_t bytes,
+                      hipStream_t stream)
+{
+    IHG_TRY(hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice,
+                                     stream),
+                      "hipMemcpyAsync"));
+    return hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
 }
 
 // filter_window / mk_filter_mask (src/setup.c:710-754), evaluated on the
@@ -254,17 +272,15 @@ static int create_ctx(const ihg_params *p, int nreal, ihg_ctx **out)
                                   (long double)m / (long double)L;
                 h[L + m] = make_double2((double)cosl(ang), (double)sinl(ang));
             }
-        if ((rc = dalloc(&c->tw, 1024)) == 0)
-            rc = hip_check(hipMemcpy(c->tw, h, 1024 * sizeof(double2),
-                                     hipMemcpyHostToDevice), "hipMemcpy");
+        if ((rc = dalloc(&c->tw, 1024, c->stream)) == 0)
+            rc = upload_now(c->tw, h, 1024 * sizeof(double2), c->stream);
         if (!rc) {
             float2 *hf = (float2 *)calloc(1024, sizeof(float2));
             for (int i = 0; i < 1024; ++i)
                 hf[i] = make_float2((float)h[i].x, (float)h[i].y);
             if ((rc = hip_check(hipMalloc(&c->twf, 1024 * sizeof(float2)),
                                 "hipMalloc")) == 0)
-                rc = hip_check(hipMemcpy(c->twf, hf, 1024 * sizeof(float2),
-                                         hipMemcpyHostToDevice), "hipMemcpy");
+                rc = upload_now(c->twf, hf, 1024 * sizeof(float2), c->stream);
             free(hf);
         }
         free(h);
@@ -273,9 +289,8 @@ static int create_ctx(const ihg_params *p, int nreal, ihg_ctx **out)
     if (p->enable_filter) {
         double *h = (double *)malloc(c->M * sizeof(double));
         host_filter_mask(c, h);
-        if ((rc = dalloc(&c->filter, c->M)) == 0)
-            rc = hip_check(hipMemcpy(c->filter, h, c->M * sizeof(double),
-                                     hipMemcpyHostToDevice), "hipMemcpy");
+        if ((rc = dalloc(&c->filter, c->M, c->stream)) == 0)
+            rc = upload_now(c->filter, h, c->M * sizeof(double), c->stream);
         free(h);
         if (rc) return fail(rc);
     }
@@ -288,18 +303,18 @@ static int create_ctx(const ihg_params *p, int nreal, ihg_ctx **out)
         c->ens.p = c->npartials;
         c->ens.s = S_NSCAL;
     }
-    if ((rc = dalloc(&c->W, 6 * c->M * nreal))) return fail(rc);
-    if ((rc = dalloc(&c->Wc, 2 * c->M))) return fail(rc);
-    if (p->enable_gw && (rc = dalloc(&c->grad3, 3 * c->N * nreal)))
+    if ((rc = dalloc(&c->W, 6 * c->M * nreal, c->stream))) return fail(rc);
+    if ((rc = dalloc(&c->Wc, 2 * c->M, c->stream))) return fail(rc);
+    if (p->enable_gw && (rc = dalloc(&c->grad3, 3 * c->N * nreal, c->stream)))
         return fail(rc);
     // rho | pressure | tmp.f in one allocation (rho and tmp.f 2N apart)
-    if ((rc = dalloc(&c->rho, 3 * c->N))) return fail(rc);
+    if ((rc = dalloc(&c->rho, 3 * c->N, c->stream))) return fail(rc);
     c->pressure = c->rho + c->N;
     c->tmpf = c->rho + 2 * c->N;
-    if ((rc = dalloc(&c->partials, c->npartials * nreal))) return fail(rc);
-    if ((rc = dalloc(&c->dscal, S_NSCAL * nreal))) return fail(rc);
+    if ((rc = dalloc(&c->partials, c->npartials * nreal, c->stream))) return fail(rc);
+    if ((rc = dalloc(&c->dscal, S_NSCAL * nreal, c->stream))) return fail(rc);
     c->obs_len = 28 + 4L * p->bins;
-    if ((rc = dalloc(&c->dobs, c->obs_len))) return fail(rc);
+    if ((rc = dalloc(&c->dobs, c->obs_len, c->stream))) return fail(rc);
     if ((rc = hip_check(hipHostMalloc((void **)&c->hscal, 16 * sizeof(double)),
                         "hipHostMalloc")))
         return fail(rc);
@@ -344,7 +359,7 @@ extern "C" int ihg_malloc(ihg_ctx *c, double **ptr, size_t count)
 {
     if (!c || !ptr) return set_error("NULL argument");
     IHG_TRY(hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize"));
-    return dalloc(ptr, count);
+    return dalloc(ptr, count, c->stream);
 }
 
 extern "C" int ihg_free(ihg_ctx *c, double *ptr)
