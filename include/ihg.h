@@ -176,7 +176,9 @@ int ihg_mk_rhs_ensemble(ihg_ctx *c, double t, double *f, double *result,
 /* HIP graphs of a context's stream: capture a sequence of library calls
  * that neither synchronises nor allocates (with the live profiler off),
  * replay it as one launch; ihg_graph_join puts independent graphs (e.g. one
- * per ensemble context) side by side in one graph. */
+ * per ensemble context) side by side in one graph.  ihg_graph_launch runs on
+ * the given context's stream only: the other joined contexts' streams must
+ * be idle (ihg_sync) before the launch and are not ordered after it. */
 typedef struct ihg_graph ihg_graph;
 int ihg_graph_begin(ihg_ctx *c);
 int ihg_graph_end(ihg_ctx *c, ihg_graph **g);

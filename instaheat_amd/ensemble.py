@@ -128,7 +128,8 @@ def run_mixed(nsteps=1000, dt=0.005, use_graph=True, seed=1, sizes=MIXED,
     if autotune:
         for e in ens:
             tuned[e.n], _ = e.autotune()
-    ih.lib().ihg_sync(ens[0].ctx)
+    for e in ens:
+        ih.lib().ihg_sync(e.ctx)
     calls0 = sum(e.calls_rhs for e in ens)
     t0 = time.perf_counter()
     run(ens, nsteps, dt, use_graph)

@@ -716,7 +716,11 @@ static int ens_step(ih_ens *e, double t, double dt)
 
 int ih_ens_capture_step(ih_ens *e, double dt, ihg_graph **out)
 {
-    int rc = ihg_graph_begin(e->g);
+    /* the captured step may be replayed on another context's stream (a
+       joined graph): the ensemble's earlier work must be finished first */
+    int rc = ihg_sync(e->g);
+    if (rc) return rc;
+    rc = ihg_graph_begin(e->g);
     if (rc) return rc;
     rc = ens_step(e, e->t, dt);
     ihg_graph *g = NULL;
@@ -734,6 +738,9 @@ int ih_ens_run_rk4(ih_ens *const *es, int n, long nsteps, double dt,
 {
     if (!es || n < 1 || nsteps < 0) return IHG_EINVAL;
     int rc = 0;
+    /* every ensemble's earlier work (initial data, observables) finished:
+       the joined graph runs on the first ensemble's stream */
+    for (int i = 0; !rc && i < n; ++i) rc = ihg_sync(es[i]->g);
     if (!use_graph) {
         for (long s = 0; !rc && s < nsteps; ++s)
             for (int i = 0; !rc && i < n; ++i) {
