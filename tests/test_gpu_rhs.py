@@ -32,6 +32,13 @@ CASES = [
     ((8, 16, 32), ANISO, 4.0, True, 0, 1.3),
     ((64, 32, 16), BOX_CONFIG, 1.0, True, 1, 1.0),
     ((32, 32, 32), BOX_CONFIG, 1.0, True, 2, 2.0),
+    # the smallest and the longest lines of every axis, and a large grid
+    ((4, 4, 4), BOX_CONFIG, 1.0, True, 3, 1.0),
+    ((512, 16, 8), ANISO, 4.0, True, 1, 1.1),
+    ((8, 512, 16), ANISO, 4.0, True, 3, 1.1),
+    ((16, 8, 512), ANISO, 4.0, True, 0, 1.1),
+    ((256, 64, 32), BOX_CONFIG, 1.0, True, 1, 1.4),
+    ((128, 128, 128), BOX_CONFIG, 1.0, True, 3, 1.2),
 ]
 
 
