@@ -39,6 +39,7 @@ CASES = [
     ((16, 8, 512), ANISO, 4.0, True, 0, 1.1),
     ((256, 64, 32), BOX_CONFIG, 1.0, True, 1, 1.4),
     ((128, 128, 128), BOX_CONFIG, 1.0, True, 3, 1.2),
+    ((256, 256, 256), BOX_CONFIG, 1.0, True, 3, 1.0),
 ]
 
 
