@@ -49,6 +49,10 @@ typedef struct ihg_params {
                               1 = GW source (products, FFTs, TT projection)
                               and GW spectrum in FP32; the state, h update and
                               everything else stay FP64 */
+    int log_bins;          /* no counterpart in the reference (linear bins):
+                              1 = the POWER_SPECTRUM_BINS bins of every
+                              spectrum of equal width in log |k|, from the
+                              fundamental |k| to k_max */
 } ihg_params;
 
 typedef struct ihg_ctx ihg_ctx;

@@ -51,6 +51,7 @@ class IhgParams(ctypes.Structure):
         ("mass", ctypes.c_double),
         ("enable_gw", ctypes.c_int), ("enable_filter", ctypes.c_int),
         ("bins", ctypes.c_int), ("gw_fp32", ctypes.c_int),
+        ("log_bins", ctypes.c_int),
     ]
 
 
@@ -324,11 +325,11 @@ def unpack_obs(obs, bins):
 
 
 def make_params(shape, bounds=((0.0, 1.0),) * 3, mass=1.0, gw=False,
-                filt=True, bins=50, gw_fp32=False):
+                filt=True, bins=50, gw_fp32=False, log_bins=False):
     nx, ny, nz = shape
     return IhgParams(nx, ny, nz, bounds[0][0], bounds[0][1], bounds[1][0],
                      bounds[1][1], bounds[2][0], bounds[2][1], mass, int(gw),
-                     int(filt), bins, int(gw_fp32))
+                     int(filt), bins, int(gw_fp32), int(log_bins))
 
 
 class DeviceArray:
@@ -370,8 +371,9 @@ class Gpu:
     """One libihg context: the reference's globals for one parameter set."""
 
     def __init__(self, shape, bounds=((0.0, 1.0),) * 3, mass=1.0, gw=False,
-                 filt=True, bins=50, gw_fp32=False):
-        self.params = make_params(shape, bounds, mass, gw, filt, bins, gw_fp32)
+                 filt=True, bins=50, gw_fp32=False, log_bins=False):
+        self.params = make_params(shape, bounds, mass, gw, filt, bins, gw_fp32,
+                                  log_bins)
         self.bins = bins
         c = _vp()
         _check(lib().ihg_create(ctypes.byref(self.params), ctypes.byref(c)))
@@ -517,9 +519,10 @@ class Sim:
                  dt=1e-7, dt_min=1e-9, max_steps=10**15, rtol=1e-10,
                  atol=1e-14, hubble_frac=1e-2, skip=1, a_initial=1.0,
                  beta=0.0, safe=0.9, minscale=0.333, maxscale=3.0,
-                 gw_fp32=False):
+                 gw_fp32=False, log_bins=False):
         cfg = default_config()
-        cfg.grid = make_params(shape, bounds, mass, gw, filt, bins, gw_fp32)
+        cfg.grid = make_params(shape, bounds, mass, gw, filt, bins, gw_fp32,
+                               log_bins)
         cfg.integration_method = method
         cfg.ti, cfg.tf, cfg.dt, cfg.dt_min = ti, tf, dt, dt_min
         cfg.max_steps = min(int(max_steps), 2**62)

@@ -251,6 +251,9 @@ static int create_ctx(const ihg_params *p, int nreal, ihg_ctx **out)
                           kg.ky2 * (c->Ny / 2) * (c->Ny / 2) +
                           kg.kz2 * (c->Nz / 2) * (c->Nz / 2);
     kg.k_max = sqrt(k2_max);
+    kg.log_bins = p->log_bins != 0;
+    kg.k_lo = fmin(kg.kx1, fmin(kg.ky1, kg.kz1));
+    kg.log_span = log(kg.k_max / kg.k_lo);
 
     int rc = 0;
     auto fail = [&](int r) {

@@ -18,6 +18,9 @@ struct KGeom {
     double kx1, ky1, kz1; // 2 pi / L_i
     double kx2, ky2, kz2; // (2 pi / L_i)^2
     double k_max;
+    // log-spaced spectrum bins: fundamental |k| and log(k_max / k_lo)
+    int log_bins;
+    double k_lo, log_span;
 
     // derivative wave numbers: Nyquist entry zeroed (kvec.x/y/z)
     __device__ __forceinline__ double kxd(int i) const

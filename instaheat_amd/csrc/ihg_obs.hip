@@ -201,10 +201,14 @@ __device__ __forceinline__ void mode_index(const KGeom &kg, long i, int &ix,
 }
 
 // bin index of src/toolbox.c:579 / :477, clamped to the histogram (with
-// many bins, rounding at the Nyquist corner could otherwise give `bins`)
-__device__ __forceinline__ int bin_of(double k, double k_max, int bins)
+// many bins, rounding at the Nyquist corner could otherwise give `bins`);
+// with log bins, bins of equal width in log |k| from the fundamental |k|
+// to k_max: trunc(bins log(k / k_lo) / log(k_max / k_lo) - 1e-14)
+__device__ __forceinline__ int bin_of(double k, const KGeom &kg, int bins)
 {
-    const int b = (int)trunc(bins * k / k_max - 1.0e-14);
+    const double x = kg.log_bins ? bins * log(k / kg.k_lo) / kg.log_span
+                                 : bins * k / kg.k_max;
+    const int b = (int)trunc(x - 1.0e-14);
     return b < 0 ? 0 : (b >= bins ? bins - 1 : b);
 }
 
@@ -225,7 +229,7 @@ __global__ __launch_bounds__(RED_WG) void k_power_spectrum(
         const double2 v = X[i];
         double p2 = v.x * v.x + v.y * v.y;
         if (fabs(kg.kzd(iz)) > DBL_EPSILON) p2 *= 2.0;
-        const int idx = bin_of(sqrt(kg.ksq(ix, iy, iz)), kg.k_max, bins);
+        const int idx = bin_of(sqrt(kg.ksq(ix, iy, iz)), kg, bins);
         atomicAdd(&hist[idx], p2 / Nd);
     }
     __syncthreads();
@@ -277,7 +281,7 @@ __global__ __launch_bounds__(RED_WG) void k_gw_spectrum(
             pw = two * (dh1r * dh1r + dh1i * dh1i + dh2r * dh2r + dh2i * dh2i);
         }
         if (fabs(kg.kzd(iz)) > DBL_EPSILON) pw *= two;
-        atomicAdd(&hist[bin_of(kd, kg.k_max, bins)], fac * k2 * k * pw);
+        atomicAdd(&hist[bin_of(kd, kg, bins)], fac * k2 * k * pw);
     }
     __syncthreads();
     double *o = part + (long)blockIdx.x * bins;

@@ -43,6 +43,7 @@ struct orc_ctx {
     double kx1, ky1, kz1, kx2, ky2, kz2; /* pars.{x,y,z}.k and .k2 */
     double *kx, *ky, *kz, *kxf, *kyf, *kzf, *ksq;
     double k2_max, k_max;
+    double k_lo, log_span;   /* log bins: fundamental |k|, log(k_max / k_lo) */
     double *filt;
     cplx *phic, *xphic, *yphic, *zphic, *psic, *fc, *deltarhoc, *work;
     double *xphi, *yphi, *zphi, *grad, *lap, *f, *deltarho;
@@ -258,6 +259,23 @@ static void mk_k_grid(orc_ctx *c)
     c->k2_max = c->kx2 * (Nx / 2) * (Nx / 2) + c->ky2 * (Ny / 2) * (Ny / 2) +
                 c->kz2 * (Nz / 2) * (Nz / 2);
     c->k_max = sqrt(c->k2_max);
+    c->k_lo = fmin(c->kx1, fmin(c->ky1, c->kz1));
+    c->log_span = log(c->k_max / c->k_lo);
+}
+
+/* Spectrum bin of a mode with |k| = k: the reference's linear bins
+ * (src/toolbox.c:579, :477), or, with log_bins, bins of equal width in
+ * log |k| from the fundamental |k| (k_lo, the smallest nonzero |k|) to
+ * k_max, index trunc(bins log(k / k_lo) / log(k_max / k_lo) - 1e-14). */
+static size_t bin_index(const orc_ctx *c, double k)
+{
+    const size_t bins = c->p.bins;
+    const double x = c->p.log_bins ? bins * log(k / c->k_lo) / c->log_span
+                                   : bins * k / c->k_max;
+    long b = (long)trunc(x - 1.0e-14);
+    if (b < 0) b = 0;
+    if (b >= (long)bins) b = (long)bins - 1;
+    return (size_t)b;
 }
 
 /* filter_window, src/setup.c:747-754 */
@@ -557,8 +575,7 @@ static void mk_power_spectrum(const orc_ctx *c, const cplx *in, double *out)
     for (size_t i = 1; i < c->M; ++i) {
         double pow2_tmp = creal(in[i] * conj(in[i]));
         if (fabs(c->kz[i]) > DBL_EPSILON) pow2_tmp *= 2.0;
-        size_t idx = (int)trunc(bins * sqrt(c->ksq[i]) / c->k_max - 1.0e-14);
-        out[idx] += pow2_tmp / c->N;
+        out[bin_index(c, sqrt(c->ksq[i]))] += pow2_tmp / c->N;
     }
 }
 
@@ -635,8 +652,7 @@ static void mk_gw_spectrum(orc_ctx *c, const double *f)
             pw = 2.0 * (dh1r * dh1r + dh1i * dh1i + dh2r * dh2r + dh2i * dh2i);
         }
         if (fabs(c->kz[i]) > DBL_EPSILON) pw *= 2.0;
-        size_t idx = (int)trunc(bins * k / c->k_max - 1.0e-14);
-        c->gw[idx] += fac * k2 * k * pw;
+        c->gw[bin_index(c, k)] += fac * k2 * k * pw;
     }
     for (size_t i = 0; i < bins; ++i) c->gw[i] /= c->N;
 }

@@ -30,6 +30,8 @@ typedef struct orc_params {
     int enable_filter;       /* ENABLE_FFT_FILTER */
     int bins;                /* POWER_SPECTRUM_BINS */
     int threads;             /* THREAD_NUMBER (<=0: OpenMP default) */
+    int log_bins;            /* 1: log-spaced spectrum bins (no counterpart
+                                in the reference, which bins linearly) */
 } orc_params;
 
 /* Observables of one output step, packed in this order:

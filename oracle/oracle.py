@@ -24,6 +24,7 @@ class OrcParams(ctypes.Structure):
         ("mass", ctypes.c_double),
         ("enable_gw", ctypes.c_int), ("enable_filter", ctypes.c_int),
         ("bins", ctypes.c_int), ("threads", ctypes.c_int),
+        ("log_bins", ctypes.c_int),
     ]
 
 
@@ -129,10 +130,10 @@ class Oracle:
     """One oracle context (the reference's globals for one parameter set)."""
 
     def __init__(self, nx, ny, nz, bounds=((0.0, 1.0),) * 3, mass=1.0,
-                 gw=False, filt=True, bins=50, threads=0):
+                 gw=False, filt=True, bins=50, threads=0, log_bins=False):
         p = OrcParams(nx, ny, nz, bounds[0][0], bounds[0][1], bounds[1][0],
                       bounds[1][1], bounds[2][0], bounds[2][1], mass, int(gw),
-                      int(filt), bins, threads)
+                      int(filt), bins, threads, int(log_bins))
         self.params = p
         self.bins = bins
         self.ctx = lib().orc_create(ctypes.byref(p))

@@ -169,3 +169,26 @@ def test_rhs_repeatable(ih):
     assert np.array_equal(r1.download(), r2.download())
     assert np.array_equal(o1[:28], o2[:28])
     assert np.allclose(o1[28:], o2[28:], rtol=1e-13, atol=0)
+
+
+@pytest.mark.parametrize("shape,gw", [((32, 16, 64), True), ((64, 64, 64), False)])
+def test_log_binned_spectra(ih, oracle_mod, shape, gw):
+    """Observables with log-spaced spectrum bins (log_bins = 1) against the
+    oracle's."""
+    bins = 32
+    y = make_state(shape, ANISO, seed=8, gw=gw, a=1.2)
+    o = oracle_mod.Oracle(*shape, bounds=ANISO, mass=4.0, gw=gw, filt=True,
+                          bins=bins, log_bins=True)
+    fo = y.copy()
+    o.mk_rhs(fo, filter=1, output=1, field=fo)
+    g = ih.Gpu(shape, bounds=ANISO, mass=4.0, gw=gw, filt=True, bins=bins,
+               log_bins=True)
+    df = g.to_device(y)
+    dr = g.alloc(g.Ntot)
+    g.mk_rhs(df, dr, flags=3)
+    og = ih.unpack_obs(g.observables(), bins)
+    oo = oracle_mod.unpack_obs(o.obs(), bins)
+    for k in ("phi_ps", "psi_ps", "rho_ps") + (("gw",) if gw else ()):
+        e = rel_l2(og[k], oo[k])
+        print("log-binned", k, e)
+        assert e < TOL_OBS, k

@@ -158,7 +158,9 @@ def test_homogeneous_rhs_is_flrw(oracle_mod):
     N = o.N
     assert np.allclose(r[:N], d[0], rtol=1e-15)
     assert np.allclose(r[N:2 * N], d[1], rtol=1e-14)
-    assert np.abs(r[2 * N:4 * N]).max() < 1e-15
+    # psi'' = p / 2 + (psi - 1/2) <p> - ...: p - <p> over O(1) values, so
+    # zero to the rounding of the mean (its order depends on the threads)
+    assert np.abs(r[2 * N:4 * N]).max() < 1e-14
     assert r[-1] == pytest.approx(d[2], rel=1e-15)
     assert np.abs(r[4 * N:-1]).max() == 0.0
 
