@@ -103,3 +103,28 @@ def test_dense_end_points(ih):
     assert rel_l2(ends[0.03], y1) < 1e-15
     sim.close()
     sim2.close()
+
+
+def test_time_based_log_binned_outputs(ih):
+    """The 64^3 GW-on DOPRI853 configuration with outputs every 0.5 time
+    units (dense output) and 64 log-spaced spectrum bins, to t = 2."""
+    n, bins = 64, 64
+    L = 2 * np.pi / 0.1
+    sim = ih.Sim((n, n, n), bounds=((0.0, L),) * 3, mass=1.0, gw=True,
+                 bins=bins, method=ih.IH_DOPRI853, ti=0.0, tf=2.0, dt=1e-7,
+                 rtol=1e-10, atol=1e-14, hubble_frac=1e-2, skip=10 ** 9,
+                 log_bins=True)
+    sim.init_gaussian(seed=2, amplitude=1e-4, k_cut=n * 0.1 / 4)
+    sim.mk_initial_psi()
+    saves = []
+    t_out = [0.5, 1.0, 1.5, 2.0]
+    sim.run_dopri853(saves=saves, t_out=t_out)
+    times = [t for t, _, _ in saves]
+    for t in t_out:
+        assert t in times
+    for t, a, obs in saves:
+        o = ih.unpack_obs(obs, bins)
+        assert np.all(np.isfinite(o["phi_ps"])) and np.all(np.isfinite(o["gw"]))
+        assert o["phi_ps"].sum() > 0
+    assert sim.t == pytest.approx(2.0)
+    sim.close()
