@@ -459,7 +459,28 @@ __global__ __launch_bounds__(WG) void k_y_tt_update_h(
     tt_component<L, P, 3>(my, j, tw, bb, kg, ix, iz, inv, o0, o1);
     load(5, bb);
     tt_component<L, P, 4>(my, j, tw, ba, kg, ix, iz, inv, o0, o1);
+    // update_h's inputs (h and hdot at the points this thread ends with;
+    // line 0's points for an idle lane, in bounds): with few points per
+    // thread they are loaded before the last transform, so their latency
+    // hides behind it (with E = 8 the registers are not there)
+    const long Nh1 = 4 * N, Nh2 = Nh1 + Next, Ndh1 = Nh2 + Next,
+               Ndh2 = Ndh1 + Next;
+    const double2 *f2 = reinterpret_cast<const double2 *>(f);
+    constexpr bool EARLY = E <= 4;
+    double2 h1v[E], h2v[E], d1v[E], d2v[E];
+    auto load_h = [&]() {
+#pragma unroll
+        for (int s = 0; s < E; ++s) {
+            const long i = base + last_point<L, P>(j, s) * st;
+            h1v[s] = f2[Nh1 / 2 + i];
+            h2v[s] = f2[Nh2 / 2 + i];
+            d1v[s] = f2[Ndh1 / 2 + i];
+            d2v[s] = f2[Ndh2 / 2 + i];
+        }
+    };
+    if constexpr (EARLY) load_h();
     tt_component<L, P, 5>(my, j, tw, bb, kg, ix, iz, inv, o0, o1);
+    if constexpr (!EARLY) load_h();
     if (!act)
         return;
     const double aa = f[Ntot - 1];
@@ -467,17 +488,13 @@ __global__ __launch_bounds__(WG) void k_y_tt_update_h(
     const double hubble = sqrt(dscal[S_RHO_MEAN] / 3.0);
     const double h3 = 3.0 * hubble;
     const double pm = dscal[S_P_MEAN];
-    const long Nh1 = 4 * N, Nh2 = Nh1 + Next, Ndh1 = Nh2 + Next,
-               Ndh2 = Ndh1 + Next;
-    const double2 *f2 = reinterpret_cast<const double2 *>(f);
     double2 *r2 = reinterpret_cast<double2 *>(result);
 #pragma unroll
     for (int s = 0; s < E; ++s) {
         const int n = last_point<L, P>(j, s);
         const long i = base + n * st;
         const double ksq = kg.ksq(ix, n, iz);
-        const double2 h1 = f2[Nh1 / 2 + i], h2 = f2[Nh2 / 2 + i];
-        const double2 d1 = f2[Ndh1 / 2 + i], d2 = f2[Ndh2 / 2 + i];
+        const double2 h1 = h1v[s], h2 = h2v[s], d1 = d1v[s], d2 = d2v[s];
         r2[Nh1 / 2 + i] = d1;
         r2[Nh2 / 2 + i] = d2;
         const double tmp = -(2.0 * pm + ksq / a2);
