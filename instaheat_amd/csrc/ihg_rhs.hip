@@ -116,14 +116,32 @@ __global__ __launch_bounds__(FFT_WG) void k_z_c2r_rho(
         const double2 z = herm_merge<L>(A, Bq, n); // in bounds for every lane
         return act ? z : make_double2(0.0, 0.0);
     };
+    // phi, dphi, psi of the points this thread finishes (K = E / 2 per
+    // thread) are loaded before the transform, so their latency hides
+    // behind it; rows past the end read row 0 (in bounds, unused)
+    constexpr int K = RPB * L / FFT_WG;
+    static_assert(K * FFT_WG == RPB * L, "whole rows per thread sweep");
+    const long row0 = (long)blockIdx.x * RPB;
+    double php[K], dphp[K], psp[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int idx = tid + k * FFT_WG;
+        const int rr = idx / L, n = idx - rr * L;
+        const long r = row0 + rr;
+        const long i = (r < nrows ? r : 0) * L + n;
+        php[k] = f[i];
+        dphp[k] = f[N + i];
+        psp[k] = f[2 * N + i];
+    }
     fft_line<L, +1, true>(lds + bc * TL::PITCH, tid % TL::T, tw, rd);
 
     const long Ntot = 4 * N + 4 * (2 * M) + 1;
     const double aa = f[Ntot - 1];
     const double a2 = aa * aa;
     double srho = 0.0, sp = 0.0;
-    const long row0 = (long)blockIdx.x * RPB;
-    for (int idx = tid; idx < RPB * L; idx += FFT_WG) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int idx = tid + k * FFT_WG;
         const int rr = idx / L, n = idx - rr * L;
         const long r = row0 + rr;
         if (r >= nrows) continue;
@@ -134,7 +152,7 @@ __global__ __launch_bounds__(FFT_WG) void k_z_c2r_rho(
         g += dy * dy;
         g += dz * dz;
         const long i = r * L + n;
-        const double ph = f[i], dph = f[N + i], ps = f[2 * N + i];
+        const double ph = php[k], dph = dphp[k], ps = psp[k];
         const double t1 = (0.5 - ps) * dph * dph;
         const double t2 = (0.5 + ps) * g / a2;
         const double V = m2 * ph * ph / 2.0;
