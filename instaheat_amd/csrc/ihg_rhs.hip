@@ -222,16 +222,26 @@ __global__ __launch_bounds__(RED_WG) void k_update_phi_psi(
     const double hubble = sqrt(dscal[S_RHO_MEAN] / 3.0);
     const double h3 = 3.0 * hubble;
     const double pm = dscal[S_P_MEAN];
+    // pairs of points (16-byte loads and stores; N is even and every block
+    // starts 16-byte aligned)
     const long stride = (long)gridDim.x * RED_WG;
-    for (long i = (long)blockIdx.x * RED_WG + threadIdx.x; i < N; i += stride) {
-        const double ph = f[i], dph = f[N + i], ps = f[2 * N + i],
-                     dps = f[3 * N + i];
-        const double lap = result[N + i], pr = result[3 * N + i];
-        result[i] = dph;
-        result[N + i] = (1.0 + 4.0 * ps) * lap / a2 - (h3 - 4.0 * dps) * dph -
-                        (1.0 + 2.0 * ps) * (m2 * ph);
-        result[2 * N + i] = dps;
-        result[3 * N + i] = 0.5 * pr + (ps - 0.5) * pm - 4.0 * hubble * dps;
+    const double2 *F = reinterpret_cast<const double2 *>(f);
+    double2 *Rs = reinterpret_cast<double2 *>(result);
+    const long H = N / 2;
+    for (long i = (long)blockIdx.x * RED_WG + threadIdx.x; i < H; i += stride) {
+        const double2 ph = F[i], dph = F[H + i], ps = F[2 * H + i],
+                      dps = F[3 * H + i];
+        const double2 lap = Rs[H + i], pr = Rs[3 * H + i];
+        Rs[i] = dph;
+        Rs[H + i] = make_double2(
+            (1.0 + 4.0 * ps.x) * lap.x / a2 - (h3 - 4.0 * dps.x) * dph.x -
+                (1.0 + 2.0 * ps.x) * (m2 * ph.x),
+            (1.0 + 4.0 * ps.y) * lap.y / a2 - (h3 - 4.0 * dps.y) * dph.y -
+                (1.0 + 2.0 * ps.y) * (m2 * ph.y));
+        Rs[2 * H + i] = dps;
+        Rs[3 * H + i] = make_double2(
+            0.5 * pr.x + (ps.x - 0.5) * pm - 4.0 * hubble * dps.x,
+            0.5 * pr.y + (ps.y - 0.5) * pm - 4.0 * hubble * dps.y);
     }
 }
 
