@@ -67,7 +67,8 @@ const char *ihg_last_error(void);
 
 /* Device memory and transfers on the context's stream (replacing
  * fftw_malloc of the state arrays, src/setup.c:354-477).  ihg_malloc
- * returns zeroed memory. */
+ * returns zeroed memory; ihg_free takes only this context's arrays, and
+ * ihg_destroy frees every array of the context still allocated. */
 int ihg_malloc(ihg_ctx *c, double **ptr, size_t count);
 int ihg_free(ihg_ctx *c, double *ptr);
 int ihg_upload(ihg_ctx *c, double *dst, const double *src, size_t count);

@@ -1,4 +1,5 @@
 // ihg_api.hip -- context set-up and the C ABI of libihg (include/ihg.h).
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -201,6 +202,8 @@ extern "C" int ihg_destroy(ihg_ctx *c)
                    c->partials, c->dscal, c->dobs};
     for (void *p : dev)
         if (p) (void)hipFree(p);
+    for (void *p : c->user_arrays)
+        (void)hipFree(p);
     if (c->hscal) (void)hipHostFree(c->hscal);
     if (c->hobs) (void)hipHostFree(c->hobs);
     (void)hipStreamDestroy(c->stream);
@@ -362,13 +365,20 @@ extern "C" int ihg_malloc(ihg_ctx *c, double **ptr, size_t count)
 {
     if (!c || !ptr) return set_error("NULL argument");
     IHG_TRY(hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize"));
-    return dalloc(ptr, count, c->stream);
+    IHG_TRY(dalloc(ptr, count, c->stream));
+    c->user_arrays.push_back(*ptr);
+    return 0;
 }
 
 extern "C" int ihg_free(ihg_ctx *c, double *ptr)
 {
     if (!c) return set_error("NULL context");
     if (!ptr) return 0;
+    auto &v = c->user_arrays;
+    auto it = std::find(v.begin(), v.end(), (void *)ptr);
+    if (it == v.end())
+        return set_error("ihg_free: %p was not allocated by this context", (void *)ptr);
+    v.erase(it);
     IHG_TRY(hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize"));
     return hip_check(hipFree(ptr), "hipFree");
 }

@@ -112,6 +112,7 @@ struct ihg_ctx {
     double *field;     // array mk_summary reads (the reference's `field`)
     int nreal;         // realizations of an ensemble context (1 otherwise)
     int cand_y = 0, cand_x = 0; // strided-pass candidates (autotuner)
+    std::vector<void *> user_arrays; // from ihg_malloc, freed by ihg_destroy
     ihg::Ens ens;      // their strides
     unsigned long calls_rhs;
     int have_obs;
