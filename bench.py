@@ -247,8 +247,8 @@ def run_gpu(args, wl):
 
 def ensemble_bench(steps):
     """The batched form: realization-steps per second of the mixed ensemble
-    as one graph launch per step against eager launches, with the ensemble
-    mean and variance of the GW spectrum per size (graph run)."""
+    with one graph launch per step and size against eager launches, with the
+    ensemble mean and variance of the GW spectrum per size (graph run)."""
     from instaheat_amd import ensemble
     out = {}
     for name, graph, tune in (("eager", False, False), ("graph", True, False),

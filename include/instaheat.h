@@ -176,8 +176,9 @@ double ih_mlp_propose(void *user, const ih_dp_record *history,
  * nreal realizations of one grid in one ensemble context (ihg.h): fixed-step
  * RK4 of run_rk4 (src/rk4.c:78-102) for all of them at once.  Several
  * ensembles (one per grid size) step together in ih_ens_run_rk4; with
- * use_graph each ensemble's step is captured once as a HIP graph and all of
- * them are replayed side by side as one graph launch per step. */
+ * use_graph each ensemble's step is captured once as a HIP graph and
+ * replayed once per step on that ensemble's own stream, the sizes side by
+ * side. */
 typedef struct ih_ens ih_ens;
 int ih_ens_create(const ih_config *cfg, int nreal, ih_ens **out);
 void ih_ens_destroy(ih_ens *e);

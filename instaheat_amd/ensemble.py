@@ -3,8 +3,8 @@
 ``Ensemble`` wraps ih_ens (include/instaheat.h): nreal realizations of one
 grid whose RHS kernels run as one launch each (the realization is the third
 grid dimension).  ``run`` steps several ensembles (one per grid size) by
-fixed-step RK4, by default as one HIP graph launch per step for all of
-them.  ``run_mixed`` is the mixed-size ensemble of 48 x 32^3 + 16 x 64^3 +
+fixed-step RK4, by default as one HIP graph launch per step and ensemble,
+each on its ensemble's own stream.  ``run_mixed`` is the mixed-size ensemble of 48 x 32^3 + 16 x 64^3 +
 4 x 128^3 realizations (independent seeds, amplitudes log-uniform in
 [1e-6, 1e-4], RK4 dt 0.005) with the ensemble mean and variance of the GW
 spectrum per size.  The reference runs one realization per process and has

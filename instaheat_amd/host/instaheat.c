@@ -749,23 +749,21 @@ int ih_ens_run_rk4(ih_ens *const *es, int n, long nsteps, double dt,
                 es[i]->calls += 4UL * es[i]->nreal;
             }
     } else {
-        /* one graph per ensemble, all of them side by side in one graph:
-           every step of every realization of every size is one launch */
+        /* one graph per ensemble, each replayed on its own ensemble's
+           stream, so the sizes run side by side as in the eager form (one
+           joined graph on a single stream ran its branches one after the
+           other: 0.58x the eager rate for 48 x 16^3 + 8 x 32^3) */
         ihg_graph **parts = calloc(n, sizeof *parts);
-        ihg_graph *all = NULL;
         if (!parts) return IHG_ENOMEM;
         for (int i = 0; !rc && i < n; ++i)
             rc = ih_ens_capture_step(es[i], dt, &parts[i]);
-        if (!rc) rc = ihg_graph_join(parts, n, &all);
         for (long s = 0; !rc && s < nsteps; ++s) {
-            rc = ihg_graph_launch(all, es[0]->g);
-            for (int i = 0; i < n; ++i) {
+            for (int i = 0; !rc && i < n; ++i) {
+                rc = ihg_graph_launch(parts[i], es[i]->g);
                 es[i]->t += dt;
                 es[i]->calls += 4UL * es[i]->nreal;
             }
         }
-        if (!rc) rc = ihg_sync(es[0]->g);
-        ihg_graph_destroy(all);
         for (int i = 0; i < n; ++i) ihg_graph_destroy(parts[i]);
         free(parts);
     }
