@@ -192,3 +192,28 @@ def test_log_binned_spectra(ih, oracle_mod, shape, gw):
         e = rel_l2(og[k], oo[k])
         print("log-binned", k, e)
         assert e < TOL_OBS, k
+
+
+def test_api_rejects_bad_arguments(ih):
+    """Misuse returns an error code with a message, and leaves the context
+    usable."""
+    import ctypes
+    L = ih.lib()
+    shape = (16, 16, 16)
+    g = ih.Gpu(shape, bounds=BOX_CONFIG, gw=True, bins=8)
+    f = g.to_device(make_state(shape, seed=1, gw=True))
+    r = g.alloc(g.Ntot)
+    assert L.ihg_mk_rhs(g.ctx, 0.0, f.ptr, f.ptr, 0) != 0           # f == result
+    assert L.ihg_mk_rhs(g.ctx, 0.0, f.ptr + 8, r.ptr, 0) != 0       # misaligned
+    assert "aligned" in L.ihg_last_error().decode()
+    assert L.ihg_init_gaussian(g.ctx, r.ptr, 1, 1e-5, 1.0, 0.0, -1.0, 1.0) != 0
+    ks = (ctypes.c_void_p * 16)(*([r.ptr] * 16))
+    assert L.ihg_dp853_dense(g.ctx, 1.5, 0.1, f.ptr, r.ptr, ks, r.ptr) != 0
+    assert L.ihg_dp853_dense_stage(g.ctx, 13, 0.1, f.ptr, ks, r.ptr) != 0
+    assert L.ihg_dp853_stage(g.ctx, 13, 0.1, f.ptr, ks, r.ptr) != 0
+    other = ih.Gpu(shape, bounds=BOX_CONFIG, gw=True, bins=8)
+    assert L.ihg_free(other.ctx, r.ptr) != 0                         # not its array
+    # still usable
+    g.mk_rhs(f, r, flags=0)
+    assert np.all(np.isfinite(r.download()))
+    other.close()
