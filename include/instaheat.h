@@ -142,6 +142,9 @@ typedef struct ih_dp_opts {
 /* the state of the latest dense output (device, Ntot doubles); valid
  * inside the save() call of a time-based output */
 double *ih_dense_state(ih_sim *s);
+/* inside a save() call: the device state whose observables are passed
+ * (field, or the dense output's state); NULL outside save() */
+const double *ih_save_state(const ih_sim *s);
 
 /* run_dopri853 (src/dopri853.c:112-158) */
 long ih_run_dopri853(ih_sim *s, const ih_dp_opts *opts, ih_save_fn save,

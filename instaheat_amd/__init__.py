@@ -146,7 +146,7 @@ HOST_SYMBOLS = [
     "ih_set_max_steps", "ih_set_tf",
     "ih_mk_rhs", "ih_init_gaussian", "ih_mk_initial_psi", "ih_read_dat",
     "ih_write_dat", "ih_run_rk4",
-    "ih_run_dopri853", "ih_dense_state", "ih_success", "ih_error_norm", "ih_proposer_features",
+    "ih_run_dopri853", "ih_dense_state", "ih_save_state", "ih_success", "ih_error_norm", "ih_proposer_features",
     "ih_mlp_propose", "ih_ens_create", "ih_ens_destroy", "ih_ens_gpu",
     "ih_ens_state", "ih_ens_stride", "ih_ens_nreal", "ih_ens_time",
     "ih_ens_calls_rhs", "ih_ens_init", "ih_ens_capture_step",
@@ -253,6 +253,7 @@ def host_lib():
         "ih_destroy": (None, [_vp]),
         "ih_gpu": (_vp, [_vp]),
         "ih_dense_state": (_vp, [_vp]),
+        "ih_save_state": (_vp, [_vp]),
         "ih_field": (_vp, [_vp]),
         "ih_dfield": (_vp, [_vp]),
         "ih_time": (d, [_vp]),
@@ -629,10 +630,17 @@ class Sim:
     def write_dat(self, path):
         _check(host_lib().ih_write_dat(self.sim, os.fsencode(path)))
 
-    def run_rk4(self, saves=None):
+    def run_rk4(self, saves=None, h5=None, on_save=None):
+        """run_rk4; `on_save(t)` runs inside every save() call; `h5`: an
+        instaheat_amd.h5.H5Output bound to this Sim that receives every
+        save() as well"""
         def cb(user, t, a, obs, n):
             if saves is not None:
                 saves.append((t, a, np.ctypeslib.as_array(obs, (n,)).copy()))
+            if on_save is not None:
+                on_save(t)
+            if h5 is not None:
+                h5.save(t, a, obs, n)
         fn = SAVE_FN(cb)
         n = host_lib().ih_run_rk4(self.sim, fn, None)
         if n < 0:
@@ -640,13 +648,15 @@ class Sim:
         return n
 
     def run_dopri853(self, forced=None, saves=None, log_cap=0, proposer=None,
-                     t_out=None, on_save=None):
+                     t_out=None, on_save=None, h5=None):
         """run_dopri853; `proposer` is a Python callable
         (history records, n, dt_classic) -> dt, or an object with a
         ``c_proposer()`` method returning (C function, user pointer), such as
         instaheat_amd.proposer.MlpProposer.  `t_out`: increasing times of
         dense (time-based) outputs; `on_save(t)` runs inside every save()
-        call (where dense_state() is the state of a time-based output)."""
+        call (where dense_state() is the state of a time-based output);
+        `h5`: an instaheat_amd.h5.H5Output bound to this Sim that receives
+        every save() as well."""
         fz = None
         if forced is not None:
             fz = np.ascontiguousarray(forced, dtype=np.float64)
@@ -674,6 +684,8 @@ class Sim:
                 saves.append((t, a, np.ctypeslib.as_array(obs, (n,)).copy()))
             if on_save is not None:
                 on_save(t)
+            if h5 is not None:
+                h5.save(t, a, obs, n)
         fn = SAVE_FN(cb)
         n = host_lib().ih_run_dopri853(self.sim, ctypes.byref(opts), fn, None,
                                        ctypes.byref(st))

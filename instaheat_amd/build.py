@@ -4,6 +4,9 @@
   ``include/ihg.h`` (hipcc, gfx950).
 * ``instaheat_amd/libinstaheat.so``: the C99 host (steppers, mk_rhs shim) of
   ``include/instaheat.h``, linked against libihg.
+* ``instaheat_amd/libinstaheat_h5.so``: the HDF5 output file of
+  ``include/instaheat_h5.h``, built when HDF5's headers and library are
+  found (``IH_HDF5_ROOT``, default ``/opt/conda``).
 * ``oracle/liboracle.so``: the CPU oracle (test infrastructure).
 
 Run ``python -m instaheat_amd.build`` or ``__graft_entry__.build()``.
@@ -34,6 +37,7 @@ C_FLAGS = ["-std=c99", "-O2", "-fPIC", "-Wall", "-Wextra",
 HIP_SOURCES = ["ihg_api.hip", "ihg_fft.hip", "ihg_rhs.hip", "ihg_obs.hip",
                "ihg_step.hip", "ihg_mlp.hip", "ihg_graph.hip", "ihg_ic.hip"]
 HOST_SOURCES = ["instaheat.c", "compat.c"]
+HDF5_ROOT = os.environ.get("IH_HDF5_ROOT", "/opt/conda")
 
 
 def _run(cmd):
@@ -75,6 +79,28 @@ def build_host():
     return lib
 
 
+def hdf5_found():
+    return (os.path.exists(os.path.join(HDF5_ROOT, "include", "hdf5.h")) and
+            os.path.exists(os.path.join(HDF5_ROOT, "lib", "libhdf5.so")))
+
+
+def build_h5():
+    """libinstaheat_h5.so, or None when HDF5 is not installed"""
+    if not hdf5_found():
+        print(f"HDF5 not found under {HDF5_ROOT}: libinstaheat_h5.so skipped")
+        return None
+    inc = os.path.join(HDF5_ROOT, "include")
+    libdir = os.path.join(HDF5_ROOT, "lib")
+    lib = os.path.join(HERE, "libinstaheat_h5.so")
+    tmp = lib + ".tmp"
+    _run([CC] + C_FLAGS + ["-I", inc, "-shared", os.path.join(HOST, "ih_h5.c"),
+                           "-L", HERE, "-linstaheat", "-lihg", "-L", libdir,
+                           "-lhdf5", "-Wl,-rpath,$ORIGIN",
+                           f"-Wl,-rpath,{libdir}", "-o", tmp])
+    os.replace(tmp, lib)
+    return lib
+
+
 def build_oracle():
     _run(["make", "-s", "-B", "-C", os.path.join(ROOT, "oracle")])
     return os.path.join(ROOT, "oracle", "liboracle.so")
@@ -82,6 +108,9 @@ def build_oracle():
 
 def build_all(oracle=True):
     libs = [build_hip(), build_host()]
+    h5 = build_h5()
+    if h5:
+        libs.append(h5)
     if oracle:
         libs.append(build_oracle())
     return libs

@@ -36,6 +36,7 @@ struct ih_sim {
        proposer's inputs); kept across runs, cleared by ih_set_time */
     ih_dp_record hist[IH_HISTORY];
     long n_hist;
+    const double *save_y; /* the state of the save() call in progress */
 };
 
 static double wall_time(void)
@@ -167,7 +168,9 @@ static int do_save(ih_sim *s, ih_save_fn save, void *user, double t)
     int rc = ihg_observables(s->g, s->obs, s->obs_len);
     if (!rc) rc = ihg_download(s->g, &a, s->field + s->Ntot - 1, 1);
     if (rc) return rc;
+    s->save_y = s->field;
     save(user, t, a, s->obs, s->obs_len);
+    s->save_y = NULL;
     return 0;
 }
 
@@ -214,13 +217,17 @@ static int dense_saves(ih_sim *s, const ih_dp_opts *o, long *next, double t0,
         double a;
         rc = ihg_observables(s->g, s->obs, s->obs_len);
         if (!rc) rc = ihg_download(s->g, &a, yd + s->Ntot - 1, 1);
-        if (!rc) save(user, o->t_out[i], a, s->obs, s->obs_len);
+        if (rc) break;
+        s->save_y = yd;
+        save(user, o->t_out[i], a, s->obs, s->obs_len);
+        s->save_y = NULL;
     }
     *next = i;
     return rc;
 }
 
 double *ih_dense_state(ih_sim *s) { return s->stage[0]; }
+const double *ih_save_state(const ih_sim *s) { return s->save_y; }
 
 /* prepare_and_save_timeslice, src/toolbox.c:96-104 */
 static int prepare_and_save(ih_sim *s, ih_save_fn save, void *user)

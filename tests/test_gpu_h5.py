@@ -1,0 +1,171 @@
+"""GPU checks of the HDF5 output file written from the host's save() hook
+(include/instaheat_h5.h, src/io.c): every slice of a DOPRI853 run with
+step-based and dense outputs holds exactly the time, a, observables, the
+strided full fields of the saved state and rho of its output call; the
+meta data and the followup snapshot; IC_FROM_H5_FILE restarts (followup,
+time slice) continue a run as if it had not stopped."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from tests.states import BOX_CONFIG, make_state
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ih():
+    import instaheat_amd
+    instaheat_amd.host_lib()
+    return instaheat_amd
+
+
+@pytest.fixture(scope="module")
+def h5():
+    from instaheat_amd import h5 as mod
+    mod.h5_lib()
+    return mod
+
+
+def _strided(y, shape, stride):
+    return y.reshape(shape)[::stride[0], ::stride[1], ::stride[2]].ravel()
+
+
+def test_dopri853_run_file_matches_saves(ih, h5, tmp_path):
+    shape, bins, stride = (16, 16, 16), 12, (2, 1, 4)
+    N = int(np.prod(shape))
+    sim = ih.Sim(shape, bounds=BOX_CONFIG, mass=4.0, gw=True, bins=bins,
+                 method=ih.IH_DOPRI853, ti=0.0, tf=10.0, skip=2)
+    y0 = make_state(shape, seed=31, gw=True, a=1.1)
+    sim.set_field(y0)
+    sim.t = 0.0
+    p = tmp_path / "run.h5"
+    w = h5.H5Output(p, sim.cfg, outputs=h5.OUT_DEFAULT | h5.OUT_FIELDS, buf_size=3,
+                    stride=stride, seed=31, sim=sim)
+    saves, states, rhos, dense = [], [], [], []
+    L, H = ih.lib(), ih.host_lib()
+
+    def grab(t):
+        states.append(sim._dl(H.ih_save_state(sim.sim), sim.Ntot))
+        r = np.empty(N)
+        assert L.ihg_aux(sim.ctx, 3, r.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) == 0
+        rhos.append(r)
+        dense.append(t in t_out)
+    dts = [0.01, 0.02, 0.02, 0.03, 0.03, 0.03, 0.04]
+    t_out = [0.015, 0.07, 0.1]
+    st = sim.run_dopri853(forced=dts, saves=saves, t_out=t_out, on_save=grab, h5=w)
+    assert w.status == 0
+    assert w.slices == len(saves) > 6
+    w.write_dp_stats(st)
+    w.write_followup()
+    final = sim.field()
+    w.close()
+    times = np.array([s[0] for s in saves])
+    assert np.array_equal(h5.read(p, "time"), times)
+    assert np.array_equal(h5.read(p, "a"), np.array([s[1] for s in saves]))
+    # the a of each slice is the saved state's (dense outputs included)
+    assert np.array_equal(h5.read(p, "a"), np.array([y[-1] for y in states]))
+    obs = np.array([s[2] for s in saves])
+    assert np.array_equal(h5.read(p, "phi_summary"), obs[:, 0:4])
+    assert np.array_equal(h5.read(p, "pressure_summary"), obs[:, 20:24])
+    assert np.array_equal(h5.read(p, "constraints"), obs[:, 24:28])
+    assert np.array_equal(h5.read(p, "gravitational_wave_spectrum"),
+                          obs[:, 28 + 3 * bins:28 + 4 * bins])
+    for j, n in enumerate(["phi", "dphi", "psi", "dpsi"]):
+        want = np.array([_strided(y[j * N:(j + 1) * N], shape, stride) for y in states])
+        assert np.array_equal(h5.read(p, n), want), n
+    want = np.array([_strided(r, shape, stride) for r in rhos])
+    assert np.array_equal(h5.read(p, "rho"), want)
+    # the phi summary of the first, last and dense-output slices describes
+    # that slice's state; those of the step-based slices in between read
+    # the state before the step, as the reference's mk_summary reads the
+    # global `field` during try_step (src/toolbox.c:634-647)
+    assert sum(dense) == len(t_out)
+    for i, y in enumerate(states):
+        if dense[i] or i in (0, len(states) - 1):
+            assert abs(obs[i, 0] - y[:N].mean()) <= 1e-12 * abs(y[:N]).max(), i
+    assert h5.read(p, "steps_total")[0] == st["n_stp"]
+    assert h5.read(p, "steps_ok")[0] == st["n_ok"]
+    assert h5.read(p, "calls_rhs")[0] == st["calls_rhs"]
+    assert np.array_equal(h5.read(p, "followup"), final)
+    sim.close()
+
+
+def _rk4(ih, shape, tf, filt=False, skip=1, dt=0.01):
+    return ih.Sim(shape, bounds=BOX_CONFIG, mass=4.0, gw=True, bins=8,
+                  method=ih.IH_RK4, ti=0.0, tf=tf, dt=dt, skip=skip, filt=filt)
+
+
+def test_followup_restart_continues_the_run(ih, h5, tmp_path):
+    """IC_FROM_H5_FILE (src/setup.c:780-781): a run stopped after 10 steps
+    with its followup snapshot, read into a new simulation and continued
+    for 10 more, ends bit-identical to the run of 20 steps that did not stop
+    (RK4 without filter: each step depends only on the state and t; dt a
+    power of two, so the times add up exactly)"""
+    shape = (16, 16, 16)
+    dt = 2.0 ** -7
+    y0 = make_state(shape, seed=8, gw=True, a=1.0)
+    a = _rk4(ih, shape, 20 * dt, dt=dt)
+    a.set_field(y0)
+    a.run_rk4()
+    want = a.field()
+    b = _rk4(ih, shape, 10 * dt, dt=dt)
+    b.set_field(y0)
+    p = tmp_path / "first.h5"
+    with h5.H5Output(p, b.cfg, sim=b) as w:
+        b.run_rk4(h5=w)
+        w.write_followup()
+    t_stop = b.t
+    assert t_stop == 10 * dt
+    assert h5.read(p, "time")[-1] == t_stop
+    c = _rk4(ih, shape, 20 * dt, dt=dt)
+    h5.read_followup(c, p)
+    assert c.t == t_stop
+    assert np.array_equal(c.field(), b.field())
+    c.run_rk4()
+    assert np.array_equal(c.field(), want)
+    # a simulation of another size refuses the snapshot
+    d = _rk4(ih, (16, 16, 32), 20 * dt, dt=dt)
+    with pytest.raises(ih.IhgError, match="specifications"):
+        h5.read_followup(d, p)
+    for s in (a, b, c, d):
+        s.close()
+
+
+def test_timeslice_initial_data(ih, h5, tmp_path):
+    """h5_read_timeslice (src/io.c:640-711): the first slice at or after t
+    (DBL_EPSILON slack), else the last; phi, dphi, psi, dpsi and a from it,
+    the metric perturbations zero"""
+    shape = (16, 8, 16)
+    N = int(np.prod(shape))
+    s = _rk4(ih, shape, 0.05)
+    s.set_field(make_state(shape, seed=4, gw=True, a=1.2))
+    p = tmp_path / "slices.h5"
+    states = []
+    with h5.H5Output(p, s.cfg, outputs=h5.OUT_DEFAULT | (h5.OUT_FIELDS & ~h5.OUT_RHO),
+                     sim=s) as w:
+        s.run_rk4(h5=w, on_save=lambda t: states.append(
+            s._dl(ih.host_lib().ih_save_state(s.sim), s.Ntot)))
+    times = h5.read(p, "time")
+    assert len(times) == len(states) == 6
+    assert np.allclose(times, [0.0, 0.01, 0.02, 0.03, 0.04, 0.05], rtol=0, atol=1e-15)
+    for t, want_i in [(0.025, 3), (times[2], 2), (0.0, 0), (9.0, 5),
+                      (times[4] + 1e-17, 4)]:
+        r = _rk4(ih, shape, 1.0)
+        got_t = h5.read_timeslice(r, p, t)
+        assert got_t == times[want_i] and r.t == times[want_i]
+        y = r.field()
+        assert np.array_equal(y[:4 * N], states[want_i][:4 * N])
+        assert y[-1] == states[want_i][-1]
+        assert not np.any(y[4 * N:-1])
+        r.close()
+    # a file without full fields at stride 1 cannot seed a simulation
+    q = tmp_path / "smry.h5"
+    with h5.H5Output(q, s.cfg) as w:
+        w.append(0.0, 1.0, np.zeros(28 + 4 * 8))
+    r = _rk4(ih, shape, 1.0)
+    with pytest.raises(ih.IhgError, match="no dataset phi"):
+        h5.read_timeslice(r, q, 0.0)
+    r.close()
+    s.close()
