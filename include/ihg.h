@@ -149,6 +149,25 @@ int ihg_dp853_dense(ihg_ctx *c, double theta, double dt, const double *y,
                     const double *y_new, const double *const *k, double *out);
 int ihg_dp853_dense_tableau(double *c_extra, double *a_extra, double *d);
 
+/* Stage arithmetic of the GSL steppers of src/gsl_stepper.c:49-58
+ * (gsl_odeiv2_step_rkf45 / _rkck / _rk8pd; method = INTEGRATION_METHOD
+ * 1 RKF45, 2 RKCK, 3 DOPRI89).  k[0] = f(t, y), k[s-1] = RHS of stage s.
+ *   ihg_gsl_stage: out = y + dt * sum_j a_{s,j} k[j]   (s = 2..stages)
+ *   ihg_gsl_finish: y_new = y + dt * sum_j b_j k[j], the embedded error
+ *     yerr = dt * sum_j e_j k[j] (RKF45, RKCK) or dt * (sum_j bhat_j k[j] -
+ *     sum_j b_j k[j]) (DOPRI89), and the standard control's ratio
+ *     *rmax = max(DBL_MIN, max_i |yerr_i| / (eps_rel |y_new_i| + eps_abs))
+ *     (control y_new: a_y = 1, a_dydt = 0).  Synchronises.
+ *   ihg_gsl_tableau: info = {stages, order, err_mode}, c[13],
+ *     a[13][12], b[13], w[13] (w = e for err_mode 0, bhat for 1). */
+int ihg_gsl_stage(ihg_ctx *c, int method, int s, double dt, const double *y,
+                  const double *const *k, double *out);
+int ihg_gsl_finish(ihg_ctx *c, int method, double dt, const double *y,
+                   const double *const *k, double *y_new, double eps_abs,
+                   double eps_rel, double *rmax);
+int ihg_gsl_tableau(int method, int info[3], double *c, double *a, double *b,
+                    double *w);
+
 /* Live timing of the kernel families (the runtime_* monitors of
  * src/main_template.h:292-311, per kernel): when enabled, every launch is
  * bracketed by HIP events on the context's stream.  ihg_prof_read returns,

@@ -25,6 +25,9 @@ extern "C" {
 
 /* INTEGRATION_METHOD values (src/main_template.h:51-56) */
 #define IH_RK4 0
+#define IH_RKF45 1
+#define IH_RKCK 2
+#define IH_DOPRI89 3
 #define IH_DOPRI853 4
 
 /* evo_flags (src/toolbox.h:16-24) */
@@ -37,7 +40,8 @@ struct ih_evolution_flags {
 typedef struct ih_config {
     ihg_params grid;          /* GRIDPOINTS_*, SPATIAL_*, MASS, ENABLE_GW,
                                  ENABLE_FFT_FILTER, POWER_SPECTRUM_BINS */
-    int integration_method;   /* INTEGRATION_METHOD (IH_RK4, IH_DOPRI853) */
+    int integration_method;   /* INTEGRATION_METHOD (IH_RK4, IH_RKF45,
+                                 IH_RKCK, IH_DOPRI89, IH_DOPRI853) */
     double ti, tf, dt;        /* INITIAL_TIME, FINAL_TIME, DELTA_T */
     double dt_min;            /* MINIMAL_DELTA_T */
     long max_steps;           /* MAX_STEPS */
@@ -149,6 +153,39 @@ const double *ih_save_state(const ih_sim *s);
 /* run_dopri853 (src/dopri853.c:112-158) */
 long ih_run_dopri853(ih_sim *s, const ih_dp_opts *opts, ih_save_fn save,
                      void *user, ih_dp_stats *stats);
+
+/* ---- GSL steppers ----------------------------------------------------
+ * run_gsl_stepper (src/gsl_stepper.c:41-76): GSL's ODE driver with the
+ * stepper of integration_method (IH_RKF45, IH_RKCK, IH_DOPRI89), start
+ * step dt, hmin = dt_min, control y_new(atol, rtol), over n_output
+ * (GSL_OUTPUT_NUMBER) intervals ending at i * (tf - ti) / n_output (as the
+ * reference computes them), prepare_and_save_timeslice() at ti and after
+ * each interval.  Within a step the filter is off, as in the reference.
+ * The driver stops at the first GSL failure (status, the reference prints
+ * it and leaves the loop); the call still returns the accepted steps. */
+#define IH_GSL_SUCCESS 0
+#define IH_GSL_EINVAL 4    /* an interval ends before the current time */
+#define IH_GSL_EMAXITER 11
+#define IH_GSL_ENOPROG 27  /* the suggested step fell below hmin */
+
+typedef struct ih_gsl_stats {
+    long steps;        /* accepted steps */
+    long attempts;     /* step attempts (GSL's evolve count) */
+    long failed_steps; /* attempts undone after the control's DEC verdict */
+    long intervals;    /* output intervals completed */
+    int status;        /* IH_GSL_* of the driver */
+    double t, h;       /* time reached, suggested next step */
+    double integration_seconds;
+    unsigned long calls_rhs; /* RHS evaluations (GSL also evaluates the
+                                RHS at the new point of every attempt,
+                                unused by this control; not made here) */
+} ih_gsl_stats;
+
+long ih_run_gsl(ih_sim *s, long n_output, ih_save_fn save, void *user,
+                ih_gsl_stats *stats);
+/* the standard control's step adjustment for the error ratio rmax: returns
+ * -1 (decrease), 0 (keep) or 1 (increase) and updates *h */
+int ih_gsl_hadjust(double rmax, double *h, int order);
 
 /* ---- learned step proposer -------------------------------------------
  * An ih_dt_proposer over libihg's MLP (no counterpart in the reference; the

@@ -8,6 +8,8 @@
  *   fft / ifft           src/toolbox.h:32-33 (device pointers here)
  *   run_rk4              src/rk4.h:13
  *   run_dopri853         src/dopri853.h:51
+ *   run_gsl_stepper      src/gsl_stepper.h:12 (GSL_OUTPUT_NUMBER from
+ *                        ih_compat_gsl_output_number, default 1000)
  *
  * ih_compat_bind() selects the simulation these act on (the reference's
  * globals pars / field / dfield); ih_compat_save_hook() installs the save()
@@ -40,6 +42,8 @@ void fft(double *in, double _Complex *out);
 void ifft(double _Complex *in, double *out);
 void run_rk4(void);
 void run_dopri853(void);
+void run_gsl_stepper(void);
+void ih_compat_gsl_output_number(long n);
 
 #ifdef __cplusplus
 }

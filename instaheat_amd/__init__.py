@@ -19,7 +19,7 @@ import numpy as np
 
 __all__ = ["Gpu", "DeviceArray", "Sim", "IhgError", "lib", "host_lib",
            "LIB_PATH", "HOST_LIB_PATH", "IHG_FILTER", "IHG_OUTPUT",
-           "IH_RK4", "IH_DOPRI853", "obs_len", "unpack_obs", "tableau",
+           "IH_RK4", "IH_RKF45", "IH_RKCK", "IH_DOPRI89", "IH_DOPRI853", "obs_len", "unpack_obs", "tableau",
            "default_config", "IhCtrl", "IhConfig"]
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -31,6 +31,9 @@ HOST_LIB_PATH = os.path.join(LIBDIR, "libinstaheat.so")
 IHG_FILTER = 1
 IHG_OUTPUT = 2
 IH_RK4 = 0
+IH_RKF45 = 1
+IH_RKCK = 2
+IH_DOPRI89 = 3
 IH_DOPRI853 = 4
 
 _dptr = ctypes.POINTER(ctypes.c_double)
@@ -97,6 +100,14 @@ PROPOSER_FN = ctypes.CFUNCTYPE(ctypes.c_double, _vp,
                                ctypes.c_double)
 
 
+class IhGslStats(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_long), ("attempts", ctypes.c_long),
+                ("failed_steps", ctypes.c_long), ("intervals", ctypes.c_long),
+                ("status", ctypes.c_int), ("t", ctypes.c_double),
+                ("h", ctypes.c_double), ("integration_seconds", ctypes.c_double),
+                ("calls_rhs", ctypes.c_ulong)]
+
+
 class IhDpOpts(ctypes.Structure):
     _fields_ = [("forced_dt", _dptr), ("n_forced", ctypes.c_long),
                 ("log", ctypes.POINTER(IhDpRecord)), ("log_cap", ctypes.c_long),
@@ -130,7 +141,8 @@ IHG_SYMBOLS = [
     "ihg_means", "ihg_observables", "ihg_calls_rhs", "ihg_mk_initial_psi",
     "ihg_rk4_stage", "ihg_rk4_final", "ihg_dp853_stage", "ihg_dp853_finish",
     "ihg_dp853_tableau", "ihg_dp853_dense_stage", "ihg_dp853_dense",
-    "ihg_dp853_dense_tableau", "ihg_aux", "ihg_prof_enable", "ihg_prof_reset",
+    "ihg_dp853_dense_tableau", "ihg_gsl_stage", "ihg_gsl_finish",
+    "ihg_gsl_tableau", "ihg_aux", "ihg_prof_enable", "ihg_prof_reset",
     "ihg_prof_read", "ihg_prof_nfam", "ihg_prof_family", "ihg_features",
     "ihg_set_gw_fp32", "ihg_init_gaussian", "ihg_autotune", "ihg_fft_candidates",
     "ihg_set_fft_candidates", "ihg_create_ensemble", "ihg_ensemble",
@@ -150,7 +162,7 @@ HOST_SYMBOLS = [
     "ih_mlp_propose", "ih_ens_create", "ih_ens_destroy", "ih_ens_gpu",
     "ih_ens_state", "ih_ens_stride", "ih_ens_nreal", "ih_ens_time",
     "ih_ens_calls_rhs", "ih_ens_init", "ih_ens_capture_step",
-    "ih_ens_run_rk4", "ih_ens_observables",
+    "ih_ens_run_rk4", "ih_ens_observables", "ih_run_gsl", "ih_gsl_hadjust",
 ]
 
 _lib = None
@@ -198,6 +210,12 @@ def lib():
         "ihg_dp853_dense": (i, [_vp, d, d, _vp, _vp, _vp, _vp]),
         "ihg_dp853_dense_tableau": (i, [_dptr] * 3),
         "ihg_aux": (i, [_vp, i, _dptr]),
+        "ihg_gsl_stage": (i, [_vp, i, i, ctypes.c_double, _vp,
+                              ctypes.POINTER(_vp), _vp]),
+        "ihg_gsl_finish": (i, [_vp, i, ctypes.c_double, _vp, ctypes.POINTER(_vp),
+                               _vp, ctypes.c_double, ctypes.c_double, _dptr]),
+        "ihg_gsl_tableau": (i, [i, ctypes.POINTER(ctypes.c_int), _dptr, _dptr,
+                                _dptr, _dptr]),
         "ihg_prof_enable": (i, [_vp, i]),
         "ihg_prof_reset": (i, [_vp]),
         "ihg_prof_read": (i, [_vp, _dptr, _dptr, ctypes.POINTER(ctypes.c_long), i]),
@@ -291,6 +309,9 @@ def host_lib():
         "ih_mlp_propose": (d, [_vp, ctypes.POINTER(IhDpRecord), ctypes.c_long,
                                d]),
         "ih_error_norm": (d, [d, d, d, ctypes.c_size_t]),
+        "ih_run_gsl": (ctypes.c_long, [_vp, ctypes.c_long, SAVE_FN, _vp,
+                                       ctypes.POINTER(IhGslStats)]),
+        "ih_gsl_hadjust": (i, [d, _dptr, i]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(H, name)
@@ -459,6 +480,18 @@ class Gpu:
         _check(lib().ihg_dp853_finish(self.ctx, dt, y.ptr, arr, ynew.ptr, rtol,
                                       atol, sums.ctypes.data_as(_dptr)))
         return sums
+
+    def gsl_stage(self, method, s, dt, y, ks, out):
+        arr = (_vp * 13)(*[k.ptr if k is not None else None for k in ks])
+        _check(lib().ihg_gsl_stage(self.ctx, method, s, dt, y.ptr, arr, out.ptr))
+
+    def gsl_finish(self, method, dt, y, ks, ynew, eps_abs, eps_rel):
+        """y_new and the standard control's error ratio"""
+        arr = (_vp * 13)(*[k.ptr if k is not None else None for k in ks])
+        r = ctypes.c_double()
+        _check(lib().ihg_gsl_finish(self.ctx, method, dt, y.ptr, arr, ynew.ptr,
+                                    eps_abs, eps_rel, ctypes.byref(r)))
+        return r.value
 
     def aux(self, which):
         out = np.zeros(self.N)
@@ -646,6 +679,24 @@ class Sim:
         if n < 0:
             _check(n)
         return n
+
+    def run_gsl(self, n_output, saves=None, on_save=None, h5=None):
+        """run_gsl_stepper with the stepper of `method` (IH_RKF45, IH_RKCK,
+        IH_DOPRI89) over n_output (GSL_OUTPUT_NUMBER) intervals; returns
+        the IhGslStats fields as a dict"""
+        def cb(user, t, a, obs, n):
+            if saves is not None:
+                saves.append((t, a, np.ctypeslib.as_array(obs, (n,)).copy()))
+            if on_save is not None:
+                on_save(t)
+            if h5 is not None:
+                h5.save(t, a, obs, n)
+        fn = SAVE_FN(cb)
+        st = IhGslStats()
+        n = host_lib().ih_run_gsl(self.sim, int(n_output), fn, None, ctypes.byref(st))
+        if n < 0:
+            _check(n)
+        return {k: getattr(st, k) for k, _ in IhGslStats._fields_}
 
     def run_dopri853(self, forced=None, saves=None, log_cap=0, proposer=None,
                      t_out=None, on_save=None, h5=None):

@@ -95,7 +95,8 @@ static void host_filter_mask(const ihg_ctx *c, double *out)
 static const char *const FAM_NAMES[F_NFAM] = {
     "z_r2c", "z_c2r", "y_pass", "x_pass", "x_fanout", "z_c2r_rho", "means",
     "update_phi_psi", "z_r2c_sources", "y_tt_update_h", "x_roundtrip",
-    "observables", "dp_stage", "dp_finish", "rk4", "stream_copy", "dp_dense"};
+    "observables", "dp_stage", "dp_finish", "rk4", "stream_copy", "dp_dense",
+    "gsl_stage", "gsl_finish"};
 
 static int prof_flush(ihg_ctx *c)
 {

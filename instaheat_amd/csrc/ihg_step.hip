@@ -10,6 +10,7 @@
 #include <cmath>
 
 #include "dp853_tableau.h"
+#include "gsl_tableau.h"
 #include "ihg_internal.h"
 #include "reduce.h"
 
@@ -280,6 +281,101 @@ __global__ __launch_bounds__(RED_WG) void k_sum2_final(
     }
 }
 
+// the GSL steppers' final sum and the standard control's error ratio
+// (gsl_odeiv2 rkf45 / rkck / rk8pd step and control y_new):
+//   y_new = y + dt sum_j b_j k_j
+//   yerr  = dt sum_j w_j k_j            (err_mode 0, w = e)
+//         = dt (sum_j w_j k_j - sum_j b_j k_j)   (err_mode 1, w = bhat)
+//   r     = max_i |yerr_i| / |eps_rel |y_new_i| + eps_abs|
+// A NaN ratio never replaces the running maximum (the control's `>`).
+struct GslFinishIn {
+    const double *k[GSL_MAX_STAGES];
+    double b[GSL_MAX_STAGES];
+    double w[GSL_MAX_STAGES];
+    int nk;
+    int err_mode;
+};
+
+__device__ __forceinline__ double gsl_point(const GslFinishIn &f, double yv,
+                                            const double *kv, double dt,
+                                            double eps_abs, double eps_rel,
+                                            double &rm)
+{
+    double sb = 0.0, sw = 0.0;
+#pragma unroll
+    for (int j = 0; j < GSL_MAX_STAGES; ++j) {
+        if (j < f.nk) {
+            sb += f.b[j] * kv[j];
+            sw += f.w[j] * kv[j];
+        }
+    }
+    const double yn = yv + dt * sb;
+    const double yerr = f.err_mode ? dt * (sw - sb) : dt * sw;
+    const double r = fabs(yerr) / fabs(eps_rel * fabs(yn) + eps_abs);
+    rm = r > rm ? r : rm;
+    return yn;
+}
+
+__global__ __launch_bounds__(RED_WG) void k_gsl_finish(
+    const double *__restrict__ y, GslFinishIn f, double *__restrict__ ynew,
+    double dt, double eps_abs, double eps_rel, long nlive, long Ntot,
+    double *__restrict__ part)
+{
+    __shared__ double sh[8];
+    double rm = 0.0;
+    const long stride = (long)gridDim.x * RED_WG;
+    const long npairs = nlive / 2;
+    const double2 *y2 = reinterpret_cast<const double2 *>(y);
+    double2 *n2 = reinterpret_cast<double2 *>(ynew);
+    for (long i = (long)blockIdx.x * RED_WG + threadIdx.x; i < npairs;
+         i += stride) {
+        double kx[GSL_MAX_STAGES], ky[GSL_MAX_STAGES];
+#pragma unroll
+        for (int j = 0; j < GSL_MAX_STAGES; ++j) {
+            if (j < f.nk) {
+                const double2 kj = reinterpret_cast<const double2 *>(f.k[j])[i];
+                kx[j] = kj.x;
+                ky[j] = kj.y;
+            }
+        }
+        const double2 yv = y2[i];
+        double2 yn;
+        yn.x = gsl_point(f, yv.x, kx, dt, eps_abs, eps_rel, rm);
+        yn.y = gsl_point(f, yv.y, ky, dt, eps_abs, eps_rel, rm);
+        n2[i] = yn;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const long i = Ntot - 1;
+        double kv[GSL_MAX_STAGES];
+        for (int j = 0; j < f.nk; ++j)
+            kv[j] = f.k[j][i];
+        ynew[i] = gsl_point(f, y[i], kv, dt, eps_abs, eps_rel, rm);
+    }
+    rm = block_reduce<OpMax>(rm, sh, RED_WG);
+    if (threadIdx.x == 0) part[blockIdx.x] = rm;
+}
+
+__global__ __launch_bounds__(RED_WG) void k_max_final(
+    const double *__restrict__ part, int nb, double *__restrict__ out)
+{
+    __shared__ double sh[8];
+    double m = 0.0;
+    for (int b = threadIdx.x; b < nb; b += RED_WG)
+        m = part[b] > m ? part[b] : m;
+    m = block_reduce<OpMax>(m, sh, RED_WG);
+    if (threadIdx.x == 0) out[0] = m;
+}
+
+static const GslTableau *gsl_tableau_of(int method)
+{
+    switch (method) {
+    case 1: return &GSL_RKF45;
+    case 2: return &GSL_RKCK;
+    case 3: return &GSL_RK8PD;
+    default: return nullptr;
+    }
+}
+
 } // namespace ihg
 
 using namespace ihg;
@@ -503,5 +599,102 @@ extern "C" int ihg_dp853_tableau(double *a, double *cc, double *b, double *bhh,
     }
     for (int j = 0; j < 3; ++j)
         bhh[j] = DP_BHH[j];
+    return 0;
+}
+
+extern "C" int ihg_gsl_tableau(int method, int info[3], double *c, double *a,
+                               double *b, double *w)
+{
+    const GslTableau *t = gsl_tableau_of(method);
+    if (!t) return set_error("GSL stepper %d is not 1 (RKF45), 2 (RKCK) or 3 (DOPRI89)",
+                             method);
+    if (!info || !c || !a || !b || !w) return set_error("NULL argument");
+    info[0] = t->stages;
+    info[1] = t->order;
+    info[2] = t->err_mode;
+    for (int s = 0; s < GSL_MAX_STAGES; ++s) {
+        c[s] = s < t->stages ? t->c[s] : 0.0;
+        b[s] = s < t->stages ? t->b[s] : 0.0;
+        w[s] = s < t->stages ? t->w[s] : 0.0;
+        for (int j = 0; j < GSL_MAX_STAGES - 1; ++j)
+            a[s * (GSL_MAX_STAGES - 1) + j] = s < t->stages ? t->a[s][j] : 0.0;
+    }
+    return 0;
+}
+
+extern "C" int ihg_gsl_stage(ihg_ctx *c, int method, int s, double dt,
+                             const double *y, const double *const *k,
+                             double *out)
+{
+    const GslTableau *t = gsl_tableau_of(method);
+    if (!c || !k) return set_error("NULL argument");
+    if (!t) return set_error("GSL stepper %d is not 1, 2 or 3", method);
+    if (s < 2 || s > t->stages)
+        return set_error("stage %d out of range 2..%d", s, t->stages);
+    IHG_TRY(check_ptr(y, "y"));
+    IHG_TRY(check_ptr(out, "out"));
+    StageIn in{};
+    int n = 0;
+    for (int j = 0; j < s - 1; ++j) {
+        if (t->a[s - 1][j] == 0.0) continue;
+        IHG_TRY(check_ptr(k[j], "k[j]"));
+        in.k[n] = k[j];
+        in.a[n] = t->a[s - 1][j];
+        ++n;
+    }
+    if (n < 1) return set_error("stage %d has no weights", s);
+    in.nk = n;
+    const long nl = live_len(c);
+    return timed(c, F_GSL_STAGE, (n + 2.0) * 8.0 * nl, [&]() {
+        hipLaunchKernelGGL(k_dp_stage, dim3(stream_blocks(nl / 2)),
+                           dim3(RED_WG), 0, c->stream, out, y, in, dt, nl,
+                           c->Ntot);
+        return hip_check(hipGetLastError(), "k_dp_stage (gsl)");
+    });
+}
+
+extern "C" int ihg_gsl_finish(ihg_ctx *c, int method, double dt,
+                              const double *y, const double *const *k,
+                              double *y_new, double eps_abs, double eps_rel,
+                              double *rmax)
+{
+    const GslTableau *t = gsl_tableau_of(method);
+    if (!c || !k || !rmax) return set_error("NULL argument");
+    if (!t) return set_error("GSL stepper %d is not 1, 2 or 3", method);
+    IHG_TRY(check_ptr(y, "y"));
+    IHG_TRY(check_ptr(y_new, "y_new"));
+    GslFinishIn in{};
+    int n = 0;
+    for (int j = 0; j < t->stages; ++j) {
+        if (t->b[j] == 0.0 && t->w[j] == 0.0) continue;
+        IHG_TRY(check_ptr(k[j], "k[j]"));
+        in.k[n] = k[j];
+        in.b[n] = t->b[j];
+        in.w[n] = t->w[j];
+        ++n;
+    }
+    in.nk = n;
+    in.err_mode = t->err_mode;
+    const long nl = live_len(c);
+    long nb = (nl / 2 + RED_WG * 4 - 1) / (RED_WG * 4);
+    if (nb > 32768) nb = 32768;
+    if (nb < 1) nb = 1;
+    if (nb + 2 > c->npartials) return set_error("partials buffer too small");
+    // y and the weighted stages in, y_new out
+    IHG_TRY(timed(c, F_GSL_FINISH, (n + 2.0) * 8.0 * nl, [&]() {
+        hipLaunchKernelGGL(k_gsl_finish, dim3((unsigned)nb), dim3(RED_WG), 0,
+                           c->stream, y, in, y_new, dt, eps_abs, eps_rel, nl,
+                           c->Ntot, c->partials);
+        return hip_check(hipGetLastError(), "k_gsl_finish");
+    }));
+    hipLaunchKernelGGL(k_max_final, dim3(1), dim3(RED_WG), 0, c->stream,
+                       c->partials, (int)nb, c->dscal + 8);
+    IHG_LAUNCHED("k_max_final");
+    IHG_HIP(hipMemcpyAsync(c->hscal + 8, c->dscal + 8, sizeof(double),
+                           hipMemcpyDeviceToHost, c->stream));
+    IHG_HIP(hipStreamSynchronize(c->stream));
+    // the control's running maximum starts at DBL_MIN
+    const double dbl_min = 2.2250738585072014e-308;
+    *rmax = c->hscal[8] > dbl_min ? c->hscal[8] : dbl_min;
     return 0;
 }

@@ -10,6 +10,7 @@ static ih_sim *g_sim = NULL;
 static ih_save_fn g_save = NULL;
 static void *g_user = NULL;
 static int g_status = 0;
+static long g_gsl_outputs = 1000; /* GSL_OUTPUT_NUMBER, src/parameters.sh:33 */
 
 void ih_compat_bind(ih_sim *s) { g_sim = s; }
 
@@ -67,5 +68,15 @@ void run_dopri853(void)
 {
     if (!bound()) return;
     long n = ih_run_dopri853(g_sim, NULL, g_save, g_user, NULL);
+    g_status = n < 0 ? (int)n : 0;
+}
+
+void ih_compat_gsl_output_number(long n) { g_gsl_outputs = n; }
+
+/* run_gsl_stepper, src/gsl_stepper.c:41-76 */
+void run_gsl_stepper(void)
+{
+    if (!bound()) return;
+    long n = ih_run_gsl(g_sim, g_gsl_outputs, g_save, g_user, NULL);
     g_status = n < 0 ? (int)n : 0;
 }

@@ -93,8 +93,8 @@ int ih_create(const ih_config *cfg, ih_sim **out)
     if (!cfg || !out) return IHG_EINVAL;
     *out = NULL;
     if (cfg->skip < 1 || cfg->dt <= 0.0 || cfg->ti > cfg->tf) return IHG_EINVAL;
-    if (cfg->integration_method != IH_RK4 &&
-        cfg->integration_method != IH_DOPRI853)
+    if (cfg->integration_method < IH_RK4 ||
+        cfg->integration_method > IH_DOPRI853)
         return IHG_EINVAL;
     ih_sim *s = calloc(1, sizeof *s);
     if (!s) return IHG_ENOMEM;
@@ -570,6 +570,132 @@ long ih_run_dopri853(ih_sim *s, const ih_dp_opts *opts, ih_save_fn save,
         stats->calls_rhs = ihg_calls_rhs(s->g);
     }
     return attempts;
+}
+
+/* ------------------------------------------------------------------------ */
+/* GSL steppers, src/gsl_stepper.c:41-76 over GSL's gsl_odeiv2 driver       */
+/* ------------------------------------------------------------------------ */
+
+/* the standard control's hadjust (gsl_odeiv2_control_y_new): decrease by
+ * S r^(-1/q), at most 5 times; increase by S r^(-1/(q+1)), at most 5 times
+ * and never below 1; S = 0.9, q the stepper's order */
+int ih_gsl_hadjust(double rmax, double *h, int order)
+{
+    const double S = 0.9;
+    const double h_old = *h;
+    if (rmax > 1.1) {
+        double r = S / pow(rmax, 1.0 / order);
+        if (r < 0.2) r = 0.2;
+        *h = r * h_old;
+        return -1;
+    }
+    if (rmax < 0.5) {
+        double r = S / pow(rmax, 1.0 / (order + 1.0));
+        if (r > 5.0) r = 5.0;
+        if (r < 1.0) r = 1.0;
+        *h = r * h_old;
+        return 1;
+    }
+    return 0;
+}
+
+/* gsl_odeiv2_evolve_apply: one accepted step from s->t towards t1 with the
+ * suggested step *h (updated).  k[0] = f(t0, y) once; a step past t1 is cut
+ * to land on t1; the control's DEC verdict undoes the attempt and retries
+ * with the smaller step unless it would not move t by one ulp. */
+static int gsl_evolve(ih_sim *s, int method, int stages, int order,
+                      const double *cc, double t1, double *h,
+                      ih_gsl_stats *st)
+{
+    const ih_config *c = &s->cfg;
+    const double t0 = s->t, dt = t1 - t0;
+    const double *k[13];
+    k[0] = s->dfield;
+    for (int j = 1; j < stages; ++j) k[j] = s->stage[j - 1];
+    int rc = ih_mk_rhs(s, t0, s->field, s->dfield);
+    double h0 = *h;
+    for (;;) {
+        int final_step = h0 > dt;
+        if (final_step) h0 = dt;
+        for (int j = 2; j <= stages && !rc; ++j) {
+            rc = ihg_gsl_stage(s->g, method, j, h0, s->field, k, s->k_tmp);
+            if (!rc) rc = ih_mk_rhs(s, t0 + cc[j - 1] * h0, s->k_tmp, s->stage[j - 2]);
+        }
+        double r = 0.0;
+        if (!rc) rc = ihg_gsl_finish(s->g, method, h0, s->field, k, s->field_new,
+                                     c->atol, c->rtol, &r);
+        if (rc) return rc;
+        ++st->attempts;
+        const double t_new = final_step ? t1 : t0 + h0;
+        const double h_old = h0;
+        if (ih_gsl_hadjust(r, &h0, order) < 0) {
+            volatile double t_curr = t_new, t_next = t_new + h0;
+            if (fabs(h0) < fabs(h_old) && t_next != t_curr) {
+                ++st->failed_steps;
+                continue;
+            }
+            h0 = h_old; /* the tolerance is out of reach: keep the step */
+        }
+        double *tmp = s->field;
+        s->field = s->field_new;
+        s->field_new = tmp;
+        ihg_set_field(s->g, s->field);
+        s->t = t_new;
+        *h = h0;
+        ++st->steps;
+        return 0;
+    }
+}
+
+long ih_run_gsl(ih_sim *s, long n_output, ih_save_fn save, void *user,
+                ih_gsl_stats *stats)
+{
+    const ih_config *c = &s->cfg;
+    const int method = c->integration_method;
+    if (method < IH_RKF45 || method > IH_DOPRI89 || n_output < 1)
+        return IHG_EINVAL;
+    if (!s->stage[11]) return IHG_EINVAL;
+    int info[3];
+    double cc[13], a[13 * 12], b[13], w[13];
+    int rc = ihg_gsl_tableau(method, info, cc, a, b, w);
+    if (rc) return rc;
+    const int stages = info[0], order = info[1];
+    ih_gsl_stats st;
+    memset(&st, 0, sizeof st);
+    double h = c->dt;
+    rc = prepare_and_save(s, save, user);
+    if (!rc) rc = ihg_sync(s->g);
+    if (rc) return rc;
+    const double t_start = wall_time();
+    const double dt_out = (c->tf - c->ti) / (double)n_output;
+    for (long i = 1; i <= n_output && st.status == IH_GSL_SUCCESS; ++i) {
+        /* gsl_odeiv2_driver_apply from s->t to t1 */
+        const double t1 = (double)i * dt_out;
+        if (t1 - s->t < 0.0) {
+            st.status = IH_GSL_EINVAL;
+            break;
+        }
+        while (t1 - s->t > 0.0) {
+            rc = gsl_evolve(s, method, stages, order, cc, t1, &h, &st);
+            if (rc) return rc;
+            if (fabs(h) < c->dt_min) {
+                st.status = IH_GSL_ENOPROG;
+                break;
+            }
+        }
+        if (st.status != IH_GSL_SUCCESS) break;
+        ++st.intervals;
+        rc = prepare_and_save(s, save, user);
+        if (rc) return rc;
+    }
+    rc = ihg_sync(s->g);
+    if (rc) return rc;
+    st.integration_seconds = wall_time() - t_start;
+    st.t = s->t;
+    st.h = h;
+    st.calls_rhs = ihg_calls_rhs(s->g);
+    if (stats) *stats = st;
+    return st.steps;
 }
 
 /* ------------------------------------------------------------------------ */

@@ -107,6 +107,7 @@ def test_host_error_norm_formula(ih):
 def test_compat_layer_exports_the_reference_names(ih):
     H = ctypes.CDLL(ih.HOST_LIB_PATH)
     for name in ["mk_rhs", "fft", "ifft", "run_rk4", "run_dopri853",
+                 "run_gsl_stepper", "ih_compat_gsl_output_number",
                  "evo_flags", "ih_compat_bind", "ih_compat_save_hook",
                  "ih_compat_status"]:
         assert hasattr(H, name), name
