@@ -31,11 +31,13 @@ published algorithm for that path:
 The Butcher tables are the published ones (Fehlberg 1969; Cash and Karp,
 ACM TOMS 16 (1990) 201; Prince and Dormand, J. Comp. Appl. Math. 7 (1981)
 67), pinned in tests/test_gsl.py by their order conditions over all rooted
-trees and by convergence rates on closed-form problems.  Which order q the
-control uses, and that GSL evaluates the RHS at the new point after every
-attempt (dydt_out, unused by this control), follow GSL's source as
-published; no GSL binary is here to compare with, so the controller's
-choices are pinned by the documented rules only.
+trees and by convergence rates on closed-form problems; the driver and
+control by the rules GSL documents.  Parity unpinned against GSL itself
+(absent here): which order q the control uses (the stepper's order, 5 / 5 /
+8) and that RK8PD's error is h (sum bhat k - sum b k) follow GSL's source as
+published, not a run of GSL.  GSL also evaluates the RHS at every
+attempt's new point (dydt_out), which this control never reads; it is not
+evaluated here.
 """
 from fractions import Fraction as F
 
