@@ -63,6 +63,10 @@ def parse():
     ap.add_argument("--copy-probe", type=float, default=0.0, metavar="GB",
                     help="only time a plain copy kernel of GB gigabytes each "
                          "way (the HBM ceiling and the counter calibration)")
+    ap.add_argument("--next-rows", action="store_true",
+                    help="only measure the follow-on paths at the workload's "
+                         "grid: device initial data, dense output, HDF5 "
+                         "saves, the GSL RK8PD stepper")
     return ap.parse_args()
 
 
@@ -318,6 +322,151 @@ def roofline(prof, grid=256):
             "kernels": kernels}
 
 
+def _fam(prof, name):
+    """(avg microseconds per launch, GB/s, fraction of the HBM peak) of one
+    live-profiler family"""
+    ms, by, cnt = prof[name]
+    if cnt == 0:
+        return None
+    gbs = by / (ms / 1e3) / 1e9
+    return {"launches": int(cnt), "avg_us": round(1e3 * ms / cnt, 2),
+            "GB_per_s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+
+def next_rows(args, wl):
+    """The follow-on paths at the workload's grid, each timed on the GPU
+    from the warm-up state of the headline run:
+      initial_data  ih_init_gaussian + mk_initial_psi (device) against
+                    numpy's restatement of the same recipe on the host;
+      dense_output  DOPRI853 with one time-based output inside every step
+                    against the same forced steps without: the extra cost
+                    per output, and k_dp_dense's HBM rate;
+      hdf5          save() into libinstaheat_h5 (default outputs, then full
+                    fields at stride 2) at the dense outputs of the same
+                    steps, against those steps without the file;
+      gsl_rk8pd     run_gsl_stepper with RK8PD (rtol 1e-10, atol 1e-14)
+                    against DOPRI853 over the same span from the same
+                    state, and the GSL kernels' HBM rates."""
+    import shutil
+    import tempfile
+    import instaheat_amd as ih
+    from instaheat_amd import h5 as ih5
+    from oracle import ic
+    n = wl["shape"][0]
+    L = ih.lib()
+    out = {"metric": f"follow-on paths at {n}^3 FP64 (GW on, filter on)"}
+    sim = make_sim(ih, wl, 10 ** 9)
+    # --- device initial data vs the host restatement ----------------------
+    reps = 5
+    L.ihg_sync(sim.ctx)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sim.init_gaussian(seed=wl["seed"], amplitude=wl["amplitude"],
+                          phi0=wl["phi0"], dphi0=wl["dphi0"], k_cut=wl["kcut"])
+        sim.mk_initial_psi()
+    ih._check(L.ihg_sync(sim.ctx))
+    gpu_s = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    ic.gaussian_state(wl["shape"], wl["bounds"], wl["mass"], wl["seed"],
+                      wl["amplitude"], wl["phi0"], wl["dphi0"], 1.0, wl["kcut"],
+                      sim.Ntot)
+    host_s = time.perf_counter() - t0
+    out["initial_data"] = {"gpu_ms": round(1e3 * gpu_s, 3),
+                           "host_numpy_ms_phi_dphi_only": round(1e3 * host_s, 1),
+                           "note": "GPU: Gaussian phi, dphi and psi, dpsi from "
+                                   "the constraints; host: phi, dphi only"}
+    print(f"initial data: {out['initial_data']}", file=sys.stderr, flush=True)
+    # --- warm-up to the error-controlled phase ------------------------------
+    sim.set_max_steps(max(args.warmup - 1, 0))
+    w = sim.run_dopri853()
+    snap = sim.snapshot()
+    nsteps = 8
+    sim.set_dt(w["dt_next"])
+    sim.set_max_steps(nsteps - 1)
+    r = sim.run_dopri853(log_cap=100)
+    dts = r["log"][r["log"][:, 7] > 0.5, 1][:nsteps]
+    t_w = snap["t"]
+
+    def forced_run(**kw):
+        sim.restore(snap)
+        L.ihg_sync(sim.ctx)
+        t0 = time.perf_counter()
+        sim.run_dopri853(forced=dts, **kw)
+        ih._check(L.ihg_sync(sim.ctx))
+        return time.perf_counter() - t0
+    # --- dense output: one output time inside every step --------------------
+    ends = t_w + np.cumsum(dts)
+    t_out = list(ends - 0.5 * dts)
+    base = min(forced_run() for _ in range(2))
+    ih.prof_enable(sim.ctx, True)
+    ih.prof_reset(sim.ctx)
+    dense = forced_run(t_out=t_out)
+    prof = ih.prof_read(sim.ctx)
+    ih.prof_enable(sim.ctx, False)
+    dense = min(dense, forced_run(t_out=t_out))
+    out["dense_output"] = {"steps": int(nsteps), "outputs": len(t_out),
+                           "ms_per_step_without": round(1e3 * base / nsteps, 2),
+                           "extra_ms_per_output": round(1e3 * (dense - base) / len(t_out), 2),
+                           "k_dp_dense": _fam(prof, "dp_dense"),
+                           "extra_stage_inputs": _fam(prof, "dp_dense_stage")}
+    print(f"dense output: {out['dense_output']}", file=sys.stderr, flush=True)
+    # --- HDF5 saves ---------------------------------------------------------
+    tmp = tempfile.mkdtemp(prefix="ih_h5_")
+    try:
+        hd = {}
+        for tag, outputs, stride in (("default_outputs", ih5.OUT_DEFAULT, (1, 1, 1)),
+                                     ("full_fields_stride2", ih5.OUT_DEFAULT | ih5.OUT_FIELDS,
+                                      (2, 2, 2))):
+            path = os.path.join(tmp, tag + ".h5")
+            with ih5.H5Output(path, sim.cfg, outputs=outputs, buf_size=1,
+                              stride=stride, sim=sim) as wr:
+                secs = forced_run(t_out=t_out, h5=wr)
+                slices = wr.slices
+            plain = forced_run(t_out=t_out, saves=[])
+            size = os.path.getsize(path)
+            os.remove(path)
+            hd[tag] = {"saves": int(slices), "file_MB": round(size / 1e6, 1),
+                       "ms_per_save": round(1e3 * (secs - plain) / slices, 2)}
+        out["hdf5"] = hd
+        print(f"hdf5: {hd}", file=sys.stderr, flush=True)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    # --- GSL RK8PD against DOPRI853 over the same span ---------------------
+    span = float(np.sum(dts))
+    sim.restore(snap)
+    y_w = sim.field()
+    g = ih.Sim(wl["shape"], bounds=wl["bounds"], mass=wl["mass"], gw=wl["gw"],
+               filt=wl["filt"], bins=wl["bins"], method=ih.IH_DOPRI89, ti=0.0,
+               tf=t_w + span, dt=float(dts[0]), dt_min=1e-9, rtol=1e-10,
+               atol=1e-14)
+    g.set_field(y_w)
+    g.t = t_w
+    ih.prof_enable(g.ctx, True)
+    ih.prof_reset(g.ctx)
+    st = g.run_gsl(1)
+    gprof = ih.prof_read(g.ctx)
+    ih.prof_enable(g.ctx, False)
+    g.set_field(y_w)
+    g.t = t_w
+    st = g.run_gsl(1)
+    g.close()
+    dp_s = forced_run()
+    out["gsl_rk8pd"] = {
+        "span": round(span, 4), "t_start": round(t_w, 4),
+        "steps": st["steps"], "attempts": st["attempts"], "rejected": st["failed_steps"],
+        "status": st["status"],
+        "sim_time_per_wall_s": round(span / st["integration_seconds"], 4),
+        "ms_per_step": round(1e3 * st["integration_seconds"] / max(st["steps"], 1), 2),
+        "dopri853_same_span_sim_time_per_wall_s": round(span / dp_s, 4),
+        "dopri853_steps": int(nsteps),
+        "k_gsl_stage": _fam(gprof, "gsl_stage"),
+        "k_gsl_finish": _fam(gprof, "gsl_finish")}
+    print(f"gsl: {out['gsl_rk8pd']}", file=sys.stderr, flush=True)
+    sim.drop(snap)
+    sim.close()
+    return out
+
+
 ORACLE_STEPS = 2
 
 
@@ -366,6 +515,9 @@ def main():
         print(json.dumps(ensemble_bench(args.steps)))
         return
     wl = workload(args.grid)
+    if args.next_rows:
+        print(json.dumps(next_rows(args, wl)))
+        return
     if args.long_run > 0:
         print(json.dumps(long_run(args, wl, args.long_run)))
         return

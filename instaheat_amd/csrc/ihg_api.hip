@@ -96,7 +96,7 @@ static const char *const FAM_NAMES[F_NFAM] = {
     "z_r2c", "z_c2r", "y_pass", "x_pass", "x_fanout", "z_c2r_rho", "means",
     "update_phi_psi", "z_r2c_sources", "y_tt_update_h", "x_roundtrip",
     "observables", "dp_stage", "dp_finish", "rk4", "stream_copy", "dp_dense",
-    "gsl_stage", "gsl_finish"};
+    "gsl_stage", "gsl_finish", "dp_dense_stage"};
 
 static int prof_flush(ihg_ctx *c)
 {

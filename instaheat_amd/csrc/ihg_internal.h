@@ -66,7 +66,7 @@ enum Fam {
     F_Z_R2C, F_Z_C2R, F_Y_PASS, F_X_PASS, F_X_FANOUT, F_Z_C2R_RHO, F_MEANS,
     F_UPDATE, F_Z_R2C_SRC, F_TT_UPDATE_H, F_X_ROUNDTRIP, F_OBS, F_DP_STAGE,
     F_DP_FINISH, F_RK4, F_STREAM_COPY, F_DP_DENSE, F_GSL_STAGE, F_GSL_FINISH,
-    F_NFAM
+    F_DP_DENSE_STAGE, F_NFAM
 };
 struct ProfRec {
     int fam;

@@ -471,7 +471,7 @@ extern "C" int ihg_dp853_dense_stage(ihg_ctx *c, int s, double dt,
     }
     in.nk = n;
     const long nl = live_len(c);
-    return timed(c, F_DP_DENSE, (n + 2.0) * 8.0 * nl, [&]() {
+    return timed(c, F_DP_DENSE_STAGE, (n + 2.0) * 8.0 * nl, [&]() {
         hipLaunchKernelGGL(k_dp_stage, dim3(stream_blocks(nl / 2)),
                            dim3(RED_WG), 0, c->stream, out, y, in, dt, nl,
                            c->Ntot);
