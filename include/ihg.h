@@ -275,6 +275,17 @@ int ihg_mlp_train(ihg_mlp *m, const float *x, const float *t, int n,
                   int epochs, int batch, float lr, unsigned long seed,
                   float *epoch_loss);
 
+/* No counterpart in the reference, whose save() copies every stride-th
+ * point on the host (LARGE_OUTPUT, src/io.c:560-598): the same sub-grid of
+ * a real field (N doubles, row-major, z fastest) gathered on the device
+ * and copied to the host, so that only the output points cross PCIe.
+ * src = a device array of N doubles, or NULL for rho of the last mk_rhs;
+ * out holds ceil(Nx/sx) ceil(Ny/sy) ceil(Nz/sz) doubles in the same order.
+ * The first strided call allocates a staging array (kept by the context).
+ * Synchronises. */
+int ihg_download_strided(ihg_ctx *c, const double *src, const int stride[3],
+                         double *out);
+
 /* Debug access to intermediate arrays of the last mk_rhs, copied to host:
  * 0..2 d_x/y/z phi (GW on; FP32 values with gw_fp32), 3 rho, 4 pressure,
  * 5 tmp.f (output calls) */

@@ -142,7 +142,7 @@ IHG_SYMBOLS = [
     "ihg_rk4_stage", "ihg_rk4_final", "ihg_dp853_stage", "ihg_dp853_finish",
     "ihg_dp853_tableau", "ihg_dp853_dense_stage", "ihg_dp853_dense",
     "ihg_dp853_dense_tableau", "ihg_gsl_stage", "ihg_gsl_finish",
-    "ihg_gsl_tableau", "ihg_aux", "ihg_prof_enable", "ihg_prof_reset",
+    "ihg_gsl_tableau", "ihg_download_strided", "ihg_aux", "ihg_prof_enable", "ihg_prof_reset",
     "ihg_prof_read", "ihg_prof_nfam", "ihg_prof_family", "ihg_features",
     "ihg_set_gw_fp32", "ihg_init_gaussian", "ihg_autotune", "ihg_fft_candidates",
     "ihg_set_fft_candidates", "ihg_create_ensemble", "ihg_ensemble",
@@ -210,6 +210,7 @@ def lib():
         "ihg_dp853_dense": (i, [_vp, d, d, _vp, _vp, _vp, _vp]),
         "ihg_dp853_dense_tableau": (i, [_dptr] * 3),
         "ihg_aux": (i, [_vp, i, _dptr]),
+        "ihg_download_strided": (i, [_vp, _vp, ctypes.POINTER(ctypes.c_int), _dptr]),
         "ihg_gsl_stage": (i, [_vp, i, i, ctypes.c_double, _vp,
                               ctypes.POINTER(_vp), _vp]),
         "ihg_gsl_finish": (i, [_vp, i, ctypes.c_double, _vp, ctypes.POINTER(_vp),

@@ -200,7 +200,7 @@ extern "C" int ihg_destroy(ihg_ctx *c)
     for (hipEvent_t e : c->ev_pool)
         (void)hipEventDestroy(e);
     void *dev[] = {c->tw, c->twf, c->filter, c->W, c->Wc, c->grad3, c->rho,
-                   c->partials, c->dscal, c->dobs};
+                   c->partials, c->dscal, c->dobs, c->gather};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     for (void *p : c->user_arrays)
@@ -611,6 +611,50 @@ extern "C" int ihg_mk_initial_psi(ihg_ctx *c, double ti, double *field,
                       "hipMemsetAsync"));
     IHG_TRY(ihg_mk_rhs(c, ti, field, dfield, 0));
     return initial_psi(c, field, dfield);
+}
+
+// the stride-th points of a real field (row-major, z fastest) packed in
+// the same order: out[(bx oy + by) oz + bz] = src[(bx sx Ny + by sy) Nz + bz sz]
+__global__ __launch_bounds__(256) void k_gather_strided(
+    const double *__restrict__ src, double *__restrict__ dst, long Ny, long Nz,
+    long oy, long oz, long total, int sx, int sy, int sz)
+{
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= total) return;
+    const long bz = i % oz;
+    const long t = i / oz;
+    const long by = t % oy;
+    const long bx = t / oy;
+    dst[i] = src[(bx * sx * Ny + by * sy) * Nz + bz * sz];
+}
+
+extern "C" int ihg_download_strided(ihg_ctx *c, const double *src,
+                                    const int stride[3], double *out)
+{
+    if (!c || !stride || !out) return set_error("NULL argument");
+    if (c->nreal > 1) return set_error("ihg_download_strided: not on an ensemble context");
+    const long n[3] = {c->Nx, c->Ny, c->Nz};
+    long o[3];
+    for (int d = 0; d < 3; ++d) {
+        if (stride[d] < 1 || stride[d] > n[d])
+            return set_error("stride[%d] = %d outside 1..%ld", d, stride[d], n[d]);
+        o[d] = (n[d] + stride[d] - 1) / stride[d];
+    }
+    if (!src) src = c->rho; // rho of the last mk_rhs
+    const long total = o[0] * o[1] * o[2];
+    if (total == c->N) return ihg_download(c, out, src, (size_t)c->N);
+    if ((size_t)total > c->gather_len) {
+        if (c->gather) IHG_HIP(hipFree(c->gather));
+        c->gather = nullptr;
+        c->gather_len = 0;
+        IHG_TRY(dalloc(&c->gather, (size_t)total, c->stream));
+        c->gather_len = (size_t)total;
+    }
+    hipLaunchKernelGGL(k_gather_strided, dim3((unsigned)((total + 255) / 256)),
+                       dim3(256), 0, c->stream, src, c->gather, n[1], n[2], o[1],
+                       o[2], total, stride[0], stride[1], stride[2]);
+    IHG_LAUNCHED("k_gather_strided");
+    return ihg_download(c, out, c->gather, (size_t)total);
 }
 
 extern "C" int ihg_aux(ihg_ctx *c, int which, double *out)

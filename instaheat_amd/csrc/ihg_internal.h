@@ -114,6 +114,8 @@ struct ihg_ctx {
     int nreal;         // realizations of an ensemble context (1 otherwise)
     int cand_y = 0, cand_x = 0; // strided-pass candidates (autotuner)
     std::vector<void *> user_arrays; // from ihg_malloc, freed by ihg_destroy
+    double *gather = nullptr; // strided-output staging (ihg_download_strided)
+    size_t gather_len = 0;
     ihg::Ens ens;      // their strides
     unsigned long calls_rhs;
     int have_obs;

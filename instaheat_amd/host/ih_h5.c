@@ -80,8 +80,8 @@ struct ih_h5 {
     int err;
     int nser;
     series ser[MAX_SERIES];
-    double *host_fields; /* 4 N, for a bound simulation's full fields */
-    double *host_rho;    /* N */
+    double *host_fields; /* 4 out_n, a bound simulation's strided fields */
+    double *host_rho;    /* out_n */
 };
 
 void ih_h5_opts_defaults(ih_h5_opts *o)
@@ -404,9 +404,10 @@ static void gather(const ih_h5 *h, const double *src, double *dst)
         }
 }
 
-/* save(), src/io.c:494-613 */
-int ih_h5_append(ih_h5 *h, double t, double a, const double *obs, size_t nobs,
-                 const double *fields, const double *rho)
+/* save(), src/io.c:494-613; `packed`: fields and rho hold only the
+ * output points already (out_n per field) */
+static int append(ih_h5 *h, double t, double a, const double *obs, size_t nobs,
+                  const double *fields, const double *rho, int packed)
 {
     if (!h || !obs) return fail(IHG_EINVAL, "NULL argument");
     if (h->err) return h->err;
@@ -423,8 +424,18 @@ int ih_h5_append(ih_h5 *h, double t, double a, const double *obs, size_t nobs,
         case SRC_TIME: dst[0] = t; break;
         case SRC_A: dst[0] = a; break;
         case SRC_OBS: memcpy(dst, obs + s->off, s->dim * sizeof(double)); break;
-        case SRC_FIELD: gather(h, fields + s->off * h->N, dst); break;
-        case SRC_RHO: gather(h, rho, dst); break;
+        case SRC_FIELD:
+            if (packed)
+                memcpy(dst, fields + s->off * h->out_n, h->out_n * sizeof(double));
+            else
+                gather(h, fields + s->off * h->N, dst);
+            break;
+        case SRC_RHO:
+            if (packed)
+                memcpy(dst, rho, h->out_n * sizeof(double));
+            else
+                gather(h, rho, dst);
+            break;
         }
     }
     ++h->slices;
@@ -438,6 +449,12 @@ int ih_h5_append(ih_h5 *h, double t, double a, const double *obs, size_t nobs,
     return rc;
 }
 
+int ih_h5_append(ih_h5 *h, double t, double a, const double *obs, size_t nobs,
+                 const double *fields, const double *rho)
+{
+    return append(h, t, a, obs, nobs, fields, rho, 0);
+}
+
 int ih_h5_bind(ih_h5 *h, ih_sim *s)
 {
     if (!h || !s) return fail(IHG_EINVAL, "NULL argument");
@@ -446,11 +463,11 @@ int ih_h5_bind(ih_h5 *h, ih_sim *s)
     if (rc) return fail(rc, "ihg_sizes: %s", ihg_last_error());
     if (sz[0] != h->N) return fail(IHG_EINVAL, "simulation has %zu points, file %zu", sz[0], h->N);
     if ((h->outputs & (IH_H5_OUT_FIELDS & ~IH_H5_OUT_RHO)) && !h->host_fields) {
-        h->host_fields = malloc(4 * h->N * sizeof(double));
+        h->host_fields = malloc(4 * h->out_n * sizeof(double));
         if (!h->host_fields) return fail(IHG_ENOMEM, "out of host memory");
     }
     if ((h->outputs & IH_H5_OUT_RHO) && !h->host_rho) {
-        h->host_rho = malloc(h->N * sizeof(double));
+        h->host_rho = malloc(h->out_n * sizeof(double));
         if (!h->host_rho) return fail(IHG_ENOMEM, "out of host memory");
     }
     h->sim = s;
@@ -465,6 +482,9 @@ void ih_h5_save(void *user, double t, double a, const double *obs, size_t nobs)
     const double *fields = NULL, *rho = NULL;
     int rc = 0;
     if (h->outputs & IH_H5_OUT_FIELDS) {
+        /* only the output points cross PCIe: gathered on the device */
+        const int stride[3] = {(int)h->stride[0], (int)h->stride[1],
+                               (int)h->stride[2]};
         if (!h->sim) rc = fail(IHG_EINVAL, "full-field output needs ih_h5_bind");
         const double *y = rc ? NULL : ih_save_state(h->sim);
         if (!rc && !y) rc = fail(IHG_EINVAL, "ih_h5_save called outside save()");
@@ -472,19 +492,19 @@ void ih_h5_save(void *user, double t, double a, const double *obs, size_t nobs)
                                          IH_H5_OUT_PSI, IH_H5_OUT_DPSI};
         for (int j = 0; j < 4 && !rc; ++j)
             if (h->outputs & bits[j]) {
-                rc = ihg_download(ih_gpu(h->sim), h->host_fields + j * h->N,
-                                  y + j * h->N, h->N);
+                rc = ihg_download_strided(ih_gpu(h->sim), y + j * h->N, stride,
+                                          h->host_fields + j * h->out_n);
                 if (rc) rc = fail(rc, "download: %s", ihg_last_error());
             }
         fields = h->host_fields;
         if (!rc && (h->outputs & IH_H5_OUT_RHO)) {
             /* rho of the output call that produced obs */
-            rc = ihg_aux(ih_gpu(h->sim), 3, h->host_rho);
+            rc = ihg_download_strided(ih_gpu(h->sim), NULL, stride, h->host_rho);
             if (rc) rc = fail(rc, "rho: %s", ihg_last_error());
             rho = h->host_rho;
         }
     }
-    if (!rc) rc = ih_h5_append(h, t, a, obs, nobs, fields, rho);
+    if (!rc) rc = append(h, t, a, obs, nobs, fields, rho, 1);
     if (rc) h->err = rc;
 }
 

@@ -169,3 +169,33 @@ def test_timeslice_initial_data(ih, h5, tmp_path):
         h5.read_timeslice(r, q, 0.0)
     r.close()
     s.close()
+
+
+def test_download_strided_matches_numpy(ih):
+    """ihg_download_strided (the device gather under full-field output)
+    against numpy slicing, rho included; bad strides are errors"""
+    shape = (16, 8, 32)
+    N = int(np.prod(shape))
+    g = ih.Gpu(shape, bounds=BOX_CONFIG, mass=4.0, gw=True, bins=8)
+    y = make_state(shape, seed=3, gw=True, a=1.0)
+    d, res = g.alloc(g.Ntot), g.alloc(g.Ntot)
+    d.upload(y)
+    g.mk_rhs(d, res, t=0.0)
+    rho = g.aux(3)
+    L = ih.lib()
+    dp = ctypes.POINTER(ctypes.c_double)
+    for stride in ((1, 1, 1), (2, 1, 4), (3, 5, 7), (16, 8, 32), (1, 8, 1)):
+        st = (ctypes.c_int * 3)(*stride)
+        want = y[N:2 * N].reshape(shape)[::stride[0], ::stride[1], ::stride[2]].ravel()
+        out = np.full(want.size, np.nan)
+        assert L.ihg_download_strided(g.ctx, d.ptr + 8 * N, st, out.ctypes.data_as(dp)) == 0
+        assert np.array_equal(out, want), stride
+        want = rho.reshape(shape)[::stride[0], ::stride[1], ::stride[2]].ravel()
+        out = np.full(want.size, np.nan)
+        assert L.ihg_download_strided(g.ctx, None, st, out.ctypes.data_as(dp)) == 0
+        assert np.array_equal(out, want), ("rho", stride)
+    out = np.zeros(N)
+    for bad in ((0, 1, 1), (1, 9, 1), (1, 1, -2)):
+        st = (ctypes.c_int * 3)(*bad)
+        assert L.ihg_download_strided(g.ctx, d.ptr, st, out.ctypes.data_as(dp)) != 0
+    g.close()
