@@ -223,6 +223,16 @@ def run_gpu(args, wl):
     prof = timed_form(ih, sim, snap, w["dt_next"], None, args.steps,
                       profile=True, forced=forms["plain"]["dts"],
                       tag="profile")["prof"]
+    # the copy ceiling at the stage kernels' footprint: a plain copy kernel
+    # of one state array (Ntot doubles each way; restore copies the
+    # snapshot into the state)
+    ih.prof_enable(sim.ctx, True)
+    ih.prof_reset(sim.ctx)
+    for _ in range(10):
+        sim.restore(snap)
+    ms, by, cnt = ih.prof_read(sim.ctx)["stream_copy"]
+    ih.prof_enable(sim.ctx, False)
+    copy_gbs = by / (ms / 1e3) / 1e9
     # the precision form (FP32 GW source and spectrum) over the plain form's
     # steps, its GW spectrum per k bin against the plain form's
     ih._check(ih.lib().ihg_set_gw_fp32(sim.ctx, 1))
@@ -246,7 +256,8 @@ def run_gpu(args, wl):
         check = dict(y=y_snap, t=t_snap, dts=dts, y_end=sim.field())
     sim.drop(snap)
     sim.close()
-    return dict(forms=forms, prof=prof, check=check, warm_t=w["t"])
+    return dict(forms=forms, prof=prof, check=check, warm_t=w["t"],
+                copy_gbs=copy_gbs)
 
 
 def ensemble_bench(steps):
@@ -295,7 +306,7 @@ def measured_traffic(kernel, grid):
     return t["traffic_bytes_per_launch"] if t.get("kernel") == "k_" + kernel else None
 
 
-def roofline(prof, grid=256):
+def roofline(prof, grid=256, copy_gbs=None):
     fams = {k: v for k, v in prof.items() if v[2] > 0}
     dom = max(fams, key=lambda k: fams[k][0])
     ms, by, cnt = fams[dom]
@@ -315,6 +326,10 @@ def roofline(prof, grid=256):
             "frac": round(ach / HBM_PEAK_GBS, 4),
             "traffic": measured_traffic(dom, grid),
             "bytes_per_launch": per_launch, "avg_launch_us": round(avg_s * 1e6, 2),
+            # the same kernel against a plain copy kernel of one state array
+            # on this GPU (the measured ceiling beside the 8 TB/s spec)
+            "copy_ceiling_GB_per_s": None if copy_gbs is None else round(copy_gbs, 1),
+            "frac_of_copy_ceiling": None if copy_gbs is None else round(ach / copy_gbs, 4),
             "fft_all_passes_GB_per_s": round(fft_by / (fft_ms / 1e3) / 1e9, 1),
             "fft_all_passes_frac": round(fft_by / (fft_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
             "fft_plain_passes_GB_per_s": round(pl_by / (pl_ms / 1e3) / 1e9, 1),
@@ -562,7 +577,7 @@ def main():
                 gw_spectrum_rel_err_median=float(np.median(gpu["forms"]["precision"]["gw_rel_err"])),
                 gw_bins_compared=len(gpu["forms"]["precision"]["gw_rel_err"])),
         },
-        "roofline": roofline(gpu["prof"], args.grid),
+        "roofline": roofline(gpu["prof"], args.grid, gpu["copy_gbs"]),
     }
     if not args.no_cpu_baseline:
         rec["cpu_baseline"] = cpu_baseline(args, wl, gpu["check"], plain["dt"])
