@@ -17,8 +17,11 @@ GSL_OUTPUT_NUMBER intervals [t, i dt], dt = (tf - ti) / GSL_OUTPUT_NUMBER,
 with prepare_and_save_timeslice() after each.  This module restates GSL's
 published algorithm for that path:
 
-* driver_apply: step until t reaches t1; after every step the suggested h
-  is capped by hmax and checked against hmin (GSL_ENOPROG below it);
+* driver_apply: GSL_EINVAL for an interval ending before t; step until t
+  reaches t1; after every step the suggested h is capped by hmax and
+  checked against hmin (GSL_ENOPROG below it).  The reference's output
+  times i (tf - ti) / n are the intended ones only for ti = 0; with ti > 0
+  the first intervals end before ti and the run stops there;
 * evolve_apply: dydt_in = f(t0, y) once per call; a step longer than the
   rest of the interval is cut to it and lands exactly on t1; the standard
   control's verdict DEC (with an actual decrease that moves t by at least
@@ -44,7 +47,7 @@ from fractions import Fraction as F
 import numpy as np
 
 RKF45, RKCK, DOPRI89 = 1, 2, 3  # INTEGRATION_METHOD, src/main_template.h:52-54
-GSL_SUCCESS, GSL_EMAXITER, GSL_ENOPROG = 0, 11, 27
+GSL_SUCCESS, GSL_EINVAL, GSL_EMAXITER, GSL_ENOPROG = 0, 4, 11, 27
 
 
 def _tab(c, a, b, w, order, err_mode):
@@ -264,7 +267,10 @@ class Driver:
             return t_new, y_new
 
     def apply(self, t, t1, y):
-        """evolve y from t to t1: (status, t, y)"""
+        """evolve y from t to t1: (status, t, y); an interval that ends
+        before t (for a positive step) is GSL_EINVAL"""
+        if t1 - t < 0.0:
+            return GSL_EINVAL, t, y
         n = 0
         while t1 - t > 0.0:
             t, y = self.evolve_apply(t, t1, y)

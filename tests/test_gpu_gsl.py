@@ -162,3 +162,25 @@ def test_gsl_methods_are_rejected_by_the_other_steppers(ih):
     with pytest.raises(ih.IhgError):
         sim.run_gsl(2)
     sim.close()
+
+
+def test_gsl_initial_time_quirk_matches_oracle(ih, oracle_mod):
+    """with INITIAL_TIME > 0 the reference's first output time
+    (tf - ti) / n lies before ti: GSL's driver returns GSL_EINVAL and the
+    run ends after the initial save, on both sides"""
+    shape = (8, 8, 8)
+    y0 = make_state(shape, seed=2, gw=False, a=1.0)
+    sim = ih.Sim(shape, bounds=BOX_CONFIG, mass=4.0, gw=False, bins=8,
+                 method=ih.IH_RKCK, ti=0.5, tf=1.0, dt=0.01)
+    sim.set_field(y0)
+    sim.t = 0.5
+    saves = []
+    st = sim.run_gsl(2, saves=saves)
+    o = oracle_mod.Oracle(*shape, bounds=BOX_CONFIG, mass=4.0, gw=False, bins=8)
+    status, t, want, d, osaves = gsl.run(o, y0, gsl.RKCK, 0.5, 1.0, 0.01, 1e-14,
+                                         1e-10, 1e-9, 2, obs=[])
+    assert st["status"] == status == gsl.GSL_EINVAL
+    assert st["steps"] == 0 == d.count
+    assert len(saves) == len(osaves) == 1
+    assert st["t"] == t == 0.5
+    sim.close()

@@ -281,6 +281,17 @@ def _conditioning(case):
     return [(p, dec, h / h_ref - 1.0) for p, dec, h in out]
 
 
+def test_driver_einval_for_an_interval_behind_t():
+    """gsl_odeiv2_driver_apply refuses t1 < t; run_gsl_stepper's output
+    times i (tf - ti) / n hit that whenever ti > 0 (src/gsl_stepper.c:60-64)"""
+    d = gsl.Driver(_f, gsl.RKCK, 1e-2, 1e-12, 1e-10)
+    y0 = _exact(0.5)
+    status, t, y = d.apply(0.5, 0.25, y0)
+    assert status == gsl.GSL_EINVAL and t == 0.5 and y is y0 and d.count == 0
+    status, t, y = d.apply(0.5, 0.5, y0)
+    assert status == gsl.GSL_SUCCESS and d.count == 0
+
+
 def test_gpu_cases_are_well_conditioned():
     """The free-running GPU parity cases make the same decisions whatever
     the rounding: with the oracle's reductions at 1 or 3 threads and its
