@@ -129,6 +129,10 @@ int main(void) {
     ih_compat_bind(sim);
     evo_flags.filter = 1;
     run_dopri853();
+    if (cfg.integration_method == IH_RKCK) {  /* never taken: links the name */
+        ih_compat_gsl_output_number(1000);
+        run_gsl_stepper();
+    }
     ih_destroy(sim);
     return ih_compat_status() ? 1 : 0;
 }
