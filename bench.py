@@ -479,7 +479,86 @@ def next_rows(args, wl):
     print(f"gsl: {out['gsl_rk8pd']}", file=sys.stderr, flush=True)
     sim.drop(snap)
     sim.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = next_rows_cpu(args)
     return out
+
+
+def next_rows_cpu(args, n=128):
+    """The oracle beside the GPU for two follow-on rows, on an n^3 sample
+    of the workload (its Python restatements stream full state vectors
+    with numpy, too slowly at 256^3 for a bounded sample), the GPU timed
+    on the same sample from the same state and step size:
+      dense_output  the extra cost of one output inside a DOPRI853 step:
+                    three extra stages, the interpolant, the output call;
+      gsl_rk8pd     one RK8PD step: 13 RHS, the stage sums, the error
+                    ratio."""
+    from scipy.integrate._ivp import dop853_coefficients as SC
+    import instaheat_amd as ih
+    from oracle import dense, gsl
+    from oracle import oracle as orc
+    wl = workload(n)
+    L = ih.lib()
+    threads = args.cpu_threads or len(os.sched_getaffinity(0))
+    sim = make_sim(ih, wl, 10 ** 9)
+    sim.set_max_steps(99)
+    w = sim.run_dopri853()
+    h = float(w["dt_next"])
+    snap = sim.snapshot()
+    y, t = sim.field(), sim.t
+
+    def gpu_dense(t_out):
+        sim.restore(snap)
+        L.ihg_sync(sim.ctx)
+        t0 = time.perf_counter()
+        sim.run_dopri853(forced=[h], t_out=t_out)
+        ih._check(L.ihg_sync(sim.ctx))
+        return time.perf_counter() - t0
+    gd = min(gpu_dense([t + 0.5 * h]) - gpu_dense(None) for _ in range(3))
+    g = ih.Sim(wl["shape"], bounds=wl["bounds"], mass=wl["mass"], gw=wl["gw"],
+               filt=wl["filt"], bins=wl["bins"], method=ih.IH_DOPRI89, ti=0.0,
+               tf=t + h, dt=h, dt_min=1e-9, rtol=1e-10, atol=1e-14)
+    gg = []
+    for _ in range(3):
+        g.set_field(y)
+        g.t = t
+        st = g.run_gsl(1)
+        gg.append(st["integration_seconds"] / max(st["attempts"], 1))
+    g.close()
+    sim.drop(snap)
+    sim.close()
+    o = orc.Oracle(n, n, n, bounds=wl["bounds"], mass=wl["mass"], gw=wl["gw"],
+                   filt=wl["filt"], bins=wl["bins"], threads=threads)
+
+    def rhs(tt, yy):
+        f = yy.copy()
+        return o.mk_rhs(f, t=tt, filter=0, output=0, field=f)
+    K, y_new = dense.step(rhs, t, y, rhs(t, y), h, SC.A, SC.B, SC.C)
+    K.append(rhs(t + h, y_new))
+    t0 = time.perf_counter()
+    for i in range(3):
+        st_ = 13 + i
+        K.append(rhs(t + SC.C[st_] * h, dense.extra_stage_input(y, K, h, SC.A[st_][:st_])))
+    yd = dense.interpolate(y, y_new, K, h, 0.5, SC.D)
+    o.mk_rhs(yd, t=t + 0.5 * h, filter=0, output=1, field=yd)
+    o.obs()
+    cd = time.perf_counter() - t0
+    del K
+    tab = gsl.tableau(gsl.DOPRI89)
+    t0 = time.perf_counter()
+    y8, e8, _ = gsl.step(rhs, t, y, rhs(t, y), h, tab)
+    gsl.rmax(y8, e8, 1e-14, 1e-10)
+    cg = time.perf_counter() - t0
+    res = {"sample": (f"{n}^3 state after 100 DOPRI853 steps (t = {t:.3f}), step "
+                      f"{h:.4g}; oracle (C99+OpenMP RHS, numpy stage sums) on "
+                      f"{threads} host threads"),
+           "cores": threads, "kind": "port",
+           "dense_output_extra_s": {"cpu": round(cd, 3), "gpu": round(gd, 5),
+                                    "gpu_over_cpu": round(cd / gd, 1)},
+           "gsl_rk8pd_step_s": {"cpu": round(cg, 3), "gpu": round(min(gg), 5),
+                                "gpu_over_cpu": round(cg / min(gg), 1)}}
+    print(f"cpu: {res}", file=sys.stderr, flush=True)
+    return res
 
 
 ORACLE_STEPS = 2
