@@ -24,7 +24,11 @@
  *     filter, gravitational_waves; strings commit_hash (optional) and
  *     method ("hyperbolic")
  *   at the end: steps_total, steps_ok, steps_bad, runtime_stepper,
- *     calls_rhs (ih_h5_write_dp_stats), followup (contiguous, Ntot).
+ *     calls_rhs (ih_h5_write_dp_stats), followup (contiguous, Ntot),
+ *     runtime_total and calls_rhs (ih_h5_close).  The ENABLE_TIMING
+ *     monitors runtime_fftw ... runtime_writeout (off by default,
+ *     src/parameters.sh:66) are not written; ih_h5_write_simple can add
+ *     them.
  * The reference's h1_summary / h2_summary ("not fully functional",
  * src/parameters.sh:89-90, off by default) are not written.
  */
@@ -167,8 +171,11 @@ int ih_h5_write_dp_stats(ih_h5 *h, const ih_dp_stats *st);
 /* h5_write_followup (src/io.c:822-827): the bound simulation's state
  * (Ntot doubles, contiguous layout) */
 int ih_h5_write_followup(ih_h5 *h);
-/* h5_close (src/io.c:472-488): write the partial buffer, close the file,
- * free h (also after an error) */
+/* h5_close (src/io.c:472-488): write the partial buffer and, as main does
+ * at the end of every run (src/main.c:99-106), runtime_total (wall
+ * seconds since ih_h5_create) and, with a bound simulation, calls_rhs,
+ * unless they were written already; close the file, free h (also after an
+ * error) */
 int ih_h5_close(ih_h5 *h);
 const char *ih_h5_last_error(void);
 

@@ -11,11 +11,13 @@
  * (src/io.c:203-256).  HDF5's own error printing is switched off inside
  * these calls; failures come back as codes with ih_h5_last_error().
  */
+#define _POSIX_C_SOURCE 199309L
 #include <float.h>
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include <hdf5.h>
 
@@ -82,7 +84,15 @@ struct ih_h5 {
     series ser[MAX_SERIES];
     double *host_fields; /* 4 out_n, a bound simulation's strided fields */
     double *host_rho;    /* out_n */
+    double t_create;     /* wall clock at ih_h5_create */
 };
+
+static double wall_time(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
 
 void ih_h5_opts_defaults(ih_h5_opts *o)
 {
@@ -296,6 +306,7 @@ int ih_h5_create(const char *path, const ih_config *cfg, const ih_h5_opts *opts,
         h->out_n *= (size_t)h->outn[d];
     }
     h->obs_len = 28 + 4 * (size_t)g->bins;
+    h->t_create = wall_time();
     h->buf_size = opts->buf_size;
     h->outputs = opts->outputs;
     const size_t bins = (size_t)g->bins;
@@ -554,7 +565,8 @@ int ih_h5_write_followup(ih_h5 *h)
     return rc;
 }
 
-/* h5_close, src/io.c:472-488 */
+/* h5_close, src/io.c:472-488, after main's last writes (src/main.c:99-106):
+ * runtime_total and calls_rhs, unless written already */
 int ih_h5_close(ih_h5 *h)
 {
     if (!h) return fail(IHG_EINVAL, "NULL argument");
@@ -562,6 +574,14 @@ int ih_h5_close(ih_h5 *h)
     quiet_on(&q);
     int rc = h->err;
     if (!rc && h->index > 0) rc = flush_buffers(h, h->index);
+    if (!rc && H5Lexists(h->file, IH_H5_RUNTIME_TOTAL_NAME, H5P_DEFAULT) == 0) {
+        const double v = wall_time() - h->t_create;
+        rc = write_compact(h->file, IH_H5_RUNTIME_TOTAL_NAME, &v, 1);
+    }
+    if (!rc && h->sim && H5Lexists(h->file, IH_H5_COUNTER_RHS, H5P_DEFAULT) == 0) {
+        const double v = (double)ihg_calls_rhs(ih_gpu(h->sim));
+        rc = write_compact(h->file, IH_H5_COUNTER_RHS, &v, 1);
+    }
     if (H5Fflush(h->file, H5F_SCOPE_GLOBAL) < 0 && !rc)
         rc = fail(IHG_EINVAL, "flush failed");
     for (int i = 0; i < h->nser; ++i) {

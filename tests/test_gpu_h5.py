@@ -118,6 +118,9 @@ def test_followup_restart_continues_the_run(ih, h5, tmp_path):
         w.write_followup()
     t_stop = b.t
     assert t_stop == 10 * dt
+    # an RK4 run's file ends with main's calls_rhs and runtime_total
+    assert h5.read(p, "calls_rhs")[0] == ih.lib().ihg_calls_rhs(b.ctx)
+    assert h5.read(p, "runtime_total")[0] > 0.0
     assert h5.read(p, "time")[-1] == t_stop
     c = _rk4(ih, shape, 20 * dt, dt=dt)
     h5.read_followup(c, p)

@@ -124,7 +124,8 @@ def test_defaults_and_gw_off(h5, tmp_path):
     p = tmp_path / "plain.h5"
     with h5.H5Output(p, cfg) as w:
         w.append(0.0, 1.0, np.arange(28 + 4 * 6, dtype=float))
-    for n in ("gravitational_wave_spectrum", "phi", "rho", "commit_hash"):
+    assert h5.read(p, "runtime_total")[0] >= 0.0
+    for n in ("gravitational_wave_spectrum", "phi", "rho", "commit_hash", "calls_rhs"):
         with pytest.raises(h5.IhgError, match="no dataset"):
             h5.dims(p, n)
     assert h5.read(p, "gravitational_waves")[0] == 0.0
@@ -224,6 +225,9 @@ def test_dp_stats_datasets(h5, tmp_path):
           "calls_rhs": 530}
     with h5.H5Output(p, cfg) as w:
         w.write_dp_stats(st)
+    # main's last writes (src/main.c:99-106): runtime_total always,
+    # calls_rhs from the stats here (no simulation bound)
+    assert h5.read(p, "runtime_total")[0] > 0.0
     assert h5.read(p, "steps_total")[0] == 41
     assert h5.read(p, "steps_ok")[0] == 38
     assert h5.read(p, "steps_bad")[0] == 3
