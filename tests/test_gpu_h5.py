@@ -202,3 +202,23 @@ def test_download_strided_matches_numpy(ih):
         st = (ctypes.c_int * 3)(*bad)
         assert L.ihg_download_strided(g.ctx, d.ptr, st, out.ctypes.data_as(dp)) != 0
     g.close()
+
+
+def test_c_host_writes_the_file(h5, tmp_path):
+    """the C host of the integration guide (tests/test_h5.py) run on the
+    GPU: DOPRI853 at 16^3 with ih_h5_save as its save() hook"""
+    import subprocess
+    from tests.test_h5 import build_c_host
+    rc, err, exe = build_c_host(tmp_path)
+    assert rc == 0, err
+    p = tmp_path / "c_run.h5"
+    r = subprocess.run([str(exe), str(p)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    t = h5.read(p, "time")
+    assert t[0] == 0.0 and t[-1] == pytest.approx(0.5)
+    n = h5.read(p, "steps_total")[0]
+    assert n > 0 and len(t) >= n / 5
+    assert h5.read(p, "phi_power_spectrum").shape == (len(t), 50)
+    assert h5.dims(p, "followup")[0] == 4 * 16 ** 3 + 4 * 2 * 16 * 16 * 9 + 1
+    assert h5.read(p, "calls_rhs")[0] > 12 * n
+    assert h5.read(p, "runtime_total")[0] > 0.0

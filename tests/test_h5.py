@@ -233,3 +233,63 @@ def test_dp_stats_datasets(h5, tmp_path):
     assert h5.read(p, "steps_bad")[0] == 3
     assert h5.read(p, "runtime_stepper")[0] == 2.5
     assert h5.read(p, "calls_rhs")[0] == 530
+
+
+C_HOST = r"""
+#include <stdio.h>
+#include "instaheat_h5.h"
+int main(int argc, char **argv) {
+    ih_config cfg; ih_config_defaults(&cfg);
+    cfg.grid.nx = cfg.grid.ny = cfg.grid.nz = 16;
+    cfg.grid.xa = cfg.grid.ya = cfg.grid.za = 0.0;
+    cfg.grid.xb = cfg.grid.yb = cfg.grid.zb = 62.83185307179586;
+    cfg.grid.mass = 1.0;
+    cfg.tf = 0.5;
+    cfg.skip = 5;
+    ih_sim *sim = NULL;
+    if (ih_create(&cfg, &sim)) { fputs(ihg_last_error(), stderr); return 1; }
+    if (ih_init_gaussian(sim, 1, 1e-4, 1.0093, -0.7137, 0.4) || ih_mk_initial_psi(sim))
+        return 1;
+    ih_h5_opts o;
+    ih_h5_opts_defaults(&o);
+    o.buf_size = 5;
+    o.seed = 1;
+    ih_h5 *h5 = NULL;
+    if (ih_h5_create(argc > 1 ? argv[1] : "run.h5", &cfg, &o, &h5)) {
+        fputs(ih_h5_last_error(), stderr);
+        return 1;
+    }
+    ih_h5_bind(h5, sim);
+    ih_dp_stats st;
+    long n = ih_run_dopri853(sim, NULL, ih_h5_save, h5, &st);
+    if (n >= 0) ih_h5_write_dp_stats(h5, &st);
+    ih_h5_write_followup(h5);
+    int rc = ih_h5_close(h5);
+    ih_destroy(sim);
+    return (n < 0 || rc) ? 1 : 0;
+}
+"""
+
+
+def build_c_host(tmp_path):
+    """compile the C host of the integration guide: DOPRI853 with the HDF5
+    writer as the save() hook; returns (returncode, stderr, binary)"""
+    import instaheat_amd as ih
+    src = tmp_path / "host.c"
+    src.write_text(C_HOST)
+    libdir = os.path.dirname(ih.HOST_LIB_PATH)
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = tmp_path / "host"
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I",
+                        os.path.join(root, "include"), str(src), "-L", libdir,
+                        "-linstaheat_h5", "-linstaheat", "-lihg",
+                        "-Wl,-rpath," + libdir, "-o", str(exe)],
+                       capture_output=True, text=True)
+    return r.returncode, r.stderr, exe
+
+
+def test_c_host_with_hdf5_output_compiles(h5, tmp_path):
+    """The C host with the HDF5 writer as the save() hook compiles and
+    links (instaheat_h5.h needs no HDF5 headers)."""
+    rc, err, _ = build_c_host(tmp_path)
+    assert rc == 0, err
