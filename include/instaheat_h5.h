@@ -173,9 +173,10 @@ int ih_h5_write_dp_stats(ih_h5 *h, const ih_dp_stats *st);
 int ih_h5_write_followup(ih_h5 *h);
 /* h5_close (src/io.c:472-488): write the partial buffer and, as main does
  * at the end of every run (src/main.c:99-106), runtime_total (wall
- * seconds since ih_h5_create) and, with a bound simulation, calls_rhs,
- * unless they were written already; close the file, free h (also after an
- * error) */
+ * seconds since ih_h5_create) and calls_rhs (the bound simulation's count
+ * at the latest ih_h5_save), unless they were written already; close the
+ * file, free h (also after an error).  It does not touch the simulation,
+ * which may be destroyed first. */
 int ih_h5_close(ih_h5 *h);
 const char *ih_h5_last_error(void);
 

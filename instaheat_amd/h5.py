@@ -131,6 +131,7 @@ class H5Output:
         o.bunch_davies_cutoff = bunch_davies_cutoff
         self._hash = commit_hash.encode() if commit_hash else None
         o.commit_hash = self._hash
+        self.h = None
         h = _vp()
         _check(h5_lib().ih_h5_create(os.fsencode(path), ctypes.byref(cfg),
                                      ctypes.byref(o), ctypes.byref(h)))
@@ -182,6 +183,14 @@ class H5Output:
         if self.h:
             h, self.h = self.h, None
             _check(h5_lib().ih_h5_close(h))
+
+    def __del__(self):
+        # an unclosed file still gets its buffered slices; errors here
+        # have nowhere to go
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def __enter__(self):
         return self

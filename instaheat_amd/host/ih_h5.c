@@ -85,6 +85,8 @@ struct ih_h5 {
     double *host_fields; /* 4 out_n, a bound simulation's strided fields */
     double *host_rho;    /* out_n */
     double t_create;     /* wall clock at ih_h5_create */
+    double calls_rhs;    /* the bound simulation's mon.calls_rhs at the
+                            latest save, < 0 before one */
 };
 
 static double wall_time(void)
@@ -307,6 +309,7 @@ int ih_h5_create(const char *path, const ih_config *cfg, const ih_h5_opts *opts,
     }
     h->obs_len = 28 + 4 * (size_t)g->bins;
     h->t_create = wall_time();
+    h->calls_rhs = -1.0;
     h->buf_size = opts->buf_size;
     h->outputs = opts->outputs;
     const size_t bins = (size_t)g->bins;
@@ -516,6 +519,9 @@ void ih_h5_save(void *user, double t, double a, const double *obs, size_t nobs)
         }
     }
     if (!rc) rc = append(h, t, a, obs, nobs, fields, rho, 1);
+    /* the count main writes at the end: the last save follows the run's
+     * last RHS call (prepare_and_save_timeslice) */
+    if (!rc && h->sim) h->calls_rhs = (double)ihg_calls_rhs(ih_gpu(h->sim));
     if (rc) h->err = rc;
 }
 
@@ -578,10 +584,10 @@ int ih_h5_close(ih_h5 *h)
         const double v = wall_time() - h->t_create;
         rc = write_compact(h->file, IH_H5_RUNTIME_TOTAL_NAME, &v, 1);
     }
-    if (!rc && h->sim && H5Lexists(h->file, IH_H5_COUNTER_RHS, H5P_DEFAULT) == 0) {
-        const double v = (double)ihg_calls_rhs(ih_gpu(h->sim));
-        rc = write_compact(h->file, IH_H5_COUNTER_RHS, &v, 1);
-    }
+    /* from the latest save: the simulation may be gone by now */
+    if (!rc && h->calls_rhs >= 0.0 &&
+        H5Lexists(h->file, IH_H5_COUNTER_RHS, H5P_DEFAULT) == 0)
+        rc = write_compact(h->file, IH_H5_COUNTER_RHS, &h->calls_rhs, 1);
     if (H5Fflush(h->file, H5F_SCOPE_GLOBAL) < 0 && !rc)
         rc = fail(IHG_EINVAL, "flush failed");
     for (int i = 0; i < h->nser; ++i) {

@@ -121,6 +121,16 @@ def test_followup_restart_continues_the_run(ih, h5, tmp_path):
     # an RK4 run's file ends with main's calls_rhs and runtime_total
     assert h5.read(p, "calls_rhs")[0] == ih.lib().ihg_calls_rhs(b.ctx)
     assert h5.read(p, "runtime_total")[0] > 0.0
+    # closing the writer after its simulation is destroyed is safe
+    e = _rk4(ih, shape, 2 * dt, dt=dt)
+    e.set_field(y0)
+    q = tmp_path / "closed_late.h5"
+    w2 = h5.H5Output(q, e.cfg, sim=e)
+    e.run_rk4(h5=w2)
+    calls = ih.lib().ihg_calls_rhs(e.ctx)
+    e.close()
+    w2.close()
+    assert h5.read(q, "calls_rhs")[0] == calls
     assert h5.read(p, "time")[-1] == t_stop
     c = _rk4(ih, shape, 20 * dt, dt=dt)
     h5.read_followup(c, p)
