@@ -142,6 +142,7 @@ class H5Output:
 
     def bind(self, sim):
         _check(h5_lib().ih_h5_bind(self.h, sim.sim))
+        self._sim = sim  # the simulation outlives the writer's use of it
 
     def append(self, t, a, obs, fields=None, rho=None):
         obs = np.ascontiguousarray(obs, dtype=np.float64)
@@ -177,6 +178,9 @@ class H5Output:
         _check(h5_lib().ih_h5_write_dp_stats(self.h, ctypes.byref(st)))
 
     def write_followup(self):
+        sim = getattr(self, "_sim", None)
+        if sim is not None and not sim.sim:
+            raise IhgError("write_followup: the bound simulation is closed")
         _check(h5_lib().ih_h5_write_followup(self.h))
 
     def close(self):
