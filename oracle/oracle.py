@@ -3,6 +3,7 @@
 Test infrastructure only: imported by tests/, ``__graft_entry__.smoke()`` and
 the CPU-baseline leg of ``bench.py``, never by the shipped package.
 """
+import contextlib
 import ctypes
 import os
 
@@ -61,46 +62,63 @@ def lib():
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             raise RuntimeError("oracle not built: run `make -C oracle`")
-        L = ctypes.CDLL(_LIB_PATH)
-        vp = ctypes.c_void_p
-        L.orc_create.restype = vp
-        L.orc_create.argtypes = [ctypes.POINTER(OrcParams)]
-        L.orc_destroy.argtypes = [vp]
-        L.orc_sizes.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
-        L.orc_threads.argtypes = [vp]
-        L.orc_threads.restype = ctypes.c_int
-        L.orc_fft_r2c.argtypes = [vp, _dptr, _dptr]
-        L.orc_fft_c2r.argtypes = [vp, _dptr, _dptr]
-        L.orc_kgrid.argtypes = [vp] + [_dptr] * 8
-        L.orc_filter_mask.argtypes = [vp, _dptr]
-        L.orc_set_field.argtypes = [vp, _dptr]
-        L.orc_mk_rhs.argtypes = [vp, ctypes.c_double, _dptr, _dptr,
-                                 ctypes.c_int, ctypes.c_int]
-        L.orc_means.argtypes = [vp, _dptr]
-        L.orc_obs.argtypes = [vp, _dptr]
-        L.orc_calls_rhs.argtypes = [vp]
-        L.orc_calls_rhs.restype = ctypes.c_ulong
-        L.orc_aux.argtypes = [vp, ctypes.c_int, _dptr]
-        L.orc_mk_initial_psi.argtypes = [vp, ctypes.c_double, _dptr, _dptr]
-        L.orc_mean.argtypes = [_dptr, ctypes.c_size_t]
-        L.orc_mean.restype = ctypes.c_double
-        L.orc_variance.argtypes = [ctypes.c_double, _dptr, ctypes.c_size_t]
-        L.orc_variance.restype = ctypes.c_double
-        L.orc_run_rk4.argtypes = [vp, _dptr, ctypes.c_double, ctypes.c_double,
-                                  ctypes.c_double, ctypes.c_long, ctypes.c_long,
-                                  SAVE_FN, vp]
-        L.orc_run_rk4.restype = ctypes.c_long
-        L.orc_run_dopri853.argtypes = [vp, _dptr, _dptr, ctypes.POINTER(DpOpts),
-                                       ctypes.POINTER(DpStats), SAVE_FN, vp,
-                                       _dptr, _dptr,
-                                       ctypes.POINTER(ctypes.c_int), ctypes.c_long]
-        L.orc_run_dopri853.restype = ctypes.c_long
-        L.orc_success.argtypes = [ctypes.POINTER(Ctrl), ctypes.POINTER(DpOpts),
-                                  ctypes.c_double, _dptr, ctypes.c_double]
-        L.orc_success.restype = ctypes.c_int
-        L.orc_dp_tableau.argtypes = [_dptr] * 5
-        _lib = L
+        _lib = _bind(ctypes.CDLL(_LIB_PATH))
     return _lib
+
+
+@contextlib.contextmanager
+def library(path):
+    """Use the oracle built at `path` (a variant compiled by a test, e.g.
+    with a planted mistake) for every call inside the block."""
+    global _lib
+    saved = _lib
+    _lib = _bind(ctypes.CDLL(os.path.abspath(path)))
+    try:
+        yield _lib
+    finally:
+        _lib = saved
+
+
+def _bind(L):
+    """argument and result types of the oracle's C functions"""
+    vp = ctypes.c_void_p
+    L.orc_create.restype = vp
+    L.orc_create.argtypes = [ctypes.POINTER(OrcParams)]
+    L.orc_destroy.argtypes = [vp]
+    L.orc_sizes.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
+    L.orc_threads.argtypes = [vp]
+    L.orc_threads.restype = ctypes.c_int
+    L.orc_fft_r2c.argtypes = [vp, _dptr, _dptr]
+    L.orc_fft_c2r.argtypes = [vp, _dptr, _dptr]
+    L.orc_kgrid.argtypes = [vp] + [_dptr] * 8
+    L.orc_filter_mask.argtypes = [vp, _dptr]
+    L.orc_set_field.argtypes = [vp, _dptr]
+    L.orc_mk_rhs.argtypes = [vp, ctypes.c_double, _dptr, _dptr,
+                             ctypes.c_int, ctypes.c_int]
+    L.orc_means.argtypes = [vp, _dptr]
+    L.orc_obs.argtypes = [vp, _dptr]
+    L.orc_calls_rhs.argtypes = [vp]
+    L.orc_calls_rhs.restype = ctypes.c_ulong
+    L.orc_aux.argtypes = [vp, ctypes.c_int, _dptr]
+    L.orc_mk_initial_psi.argtypes = [vp, ctypes.c_double, _dptr, _dptr]
+    L.orc_mean.argtypes = [_dptr, ctypes.c_size_t]
+    L.orc_mean.restype = ctypes.c_double
+    L.orc_variance.argtypes = [ctypes.c_double, _dptr, ctypes.c_size_t]
+    L.orc_variance.restype = ctypes.c_double
+    L.orc_run_rk4.argtypes = [vp, _dptr, ctypes.c_double, ctypes.c_double,
+                              ctypes.c_double, ctypes.c_long, ctypes.c_long,
+                              SAVE_FN, vp]
+    L.orc_run_rk4.restype = ctypes.c_long
+    L.orc_run_dopri853.argtypes = [vp, _dptr, _dptr, ctypes.POINTER(DpOpts),
+                                   ctypes.POINTER(DpStats), SAVE_FN, vp,
+                                   _dptr, _dptr,
+                                   ctypes.POINTER(ctypes.c_int), ctypes.c_long]
+    L.orc_run_dopri853.restype = ctypes.c_long
+    L.orc_success.argtypes = [ctypes.POINTER(Ctrl), ctypes.POINTER(DpOpts),
+                              ctypes.c_double, _dptr, ctypes.c_double]
+    L.orc_success.restype = ctypes.c_int
+    L.orc_dp_tableau.argtypes = [_dptr] * 5
+    return L
 
 
 def _p(a):
@@ -136,7 +154,8 @@ class Oracle:
                       int(filt), bins, threads, int(log_bins))
         self.params = p
         self.bins = bins
-        self.ctx = lib().orc_create(ctypes.byref(p))
+        self._L = lib()  # the library that owns ctx (see library())
+        self.ctx = self._L.orc_create(ctypes.byref(p))
         if not self.ctx:
             raise ValueError("orc_create rejected the parameters")
         s = (ctypes.c_size_t * 4)()
@@ -148,7 +167,7 @@ class Oracle:
 
     def __del__(self):
         if getattr(self, "ctx", None):
-            lib().orc_destroy(self.ctx)
+            self._L.orc_destroy(self.ctx)
             self.ctx = None
 
     @property
