@@ -6,11 +6,12 @@ phi0 = 1.0093, dphi0 = -0.7137, a0 = 1, seeded Gaussian fluctuations of
 amplitude 1e-5 (|dphi_k|^2 = 1/(2 a^3 omega_k), cut at N k_min / 4), psi from
 the Einstein constraint, observables every TIME_STEP_SKIPS = 10 steps.
 
-A "step" is one accepted DOPRI853 step (perform_step).  W warm-up steps of
-the classic controller take the run from DELTA_T through the Hubble-capped
-phase (where every controller takes the capped step) into the phase where
-the error estimate sets the step.  From that state (a device snapshot) each
-form takes 8 untimed steps, filling the step history, then K timed steps:
+A "step" is one accepted DOPRI853 step (perform_step).  An untimed pre-roll
+of the classic controller takes the run from DELTA_T at t = 0 to t = 23,
+through the Hubble-capped phase (where every controller takes the capped
+step) into the phase where the error estimate sets the step; W warm-up
+steps follow.  From that state (a device snapshot) each form takes 8
+untimed steps, filling the step history, then K timed steps:
   plain  success() alone (the reference's controller);
   tuned  the learned step proposer (instaheat_amd/proposer_model.json)
          before success(); the embedded error still judges every step.
@@ -20,7 +21,9 @@ value = the tuned form's simulated time advanced / wall seconds of its K
 timed steps (the mon.integration span), every array resident in HBM; the
 plain form's figures are reported beside it.  The per-kernel figures of the
 roofline come from one more replay of the plain form's steps under the
-live profiler.
+live profiler.  Both forms also run the whole span t = 0 to 200 from the
+initial data (RHS evaluations, rejected steps, integration seconds), which
+is reported beside the timed window (--no-long-run skips it).
 
 Prints ONE JSON line.  --gpus > 1 is refused: this build runs on one GPU.
 """
@@ -48,10 +51,12 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--skip", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-long-run", action="store_true",
+                    help="leave out the t = 0 to 200 comparison of both forms")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--ensemble", action="store_true",
                     help="only run the mixed-size ensemble (48 x 32^3 + 16 x "
@@ -91,6 +96,10 @@ def copy_probe(gb, reps=10):
 
 
 HISTORY_STEPS = 8
+# simulated time of the untimed pre-roll: past the Hubble-capped phase of
+# the 256^3 workload, so that the error estimate sets the timed steps
+PRE_ROLL_T = 23.0
+LONG_RUN_T = 200.0
 
 
 def workload(n):
@@ -203,11 +212,20 @@ def long_run(args, wl, tf):
 def run_gpu(args, wl):
     import instaheat_amd as ih
     from instaheat_amd.proposer import MlpProposer
-    sim = make_sim(ih, wl, args.skip)
-    # warm-up: W accepted steps (the loop takes max_steps + 1 steps)
-    sim.set_max_steps(max(args.warmup - 1, 0))
-    w = sim.run_dopri853()
-    print(f"warm-up: {w['n_ok']} + {w['n_bad']} rejected steps to t = {w['t']:.4f}, "
+    sim = make_sim(ih, wl, args.skip, tf=PRE_ROLL_T)
+    # pre-roll to t = PRE_ROLL_T, then W warm-up steps (the loop takes
+    # max_steps + 1 steps)
+    pre = sim.run_dopri853()
+    print(f"pre-roll: {pre['n_ok']} + {pre['n_bad']} rejected steps to "
+          f"t = {pre['t']:.4f}, dt_next {pre['dt_next']:.4g}", file=sys.stderr,
+          flush=True)
+    sim.set_tf(1e9)
+    sim.set_dt(pre["dt_next"])
+    w = {"dt_next": pre["dt_next"], "t": sim.t}
+    if args.warmup > 0:
+        sim.set_max_steps(args.warmup - 1)
+        w = sim.run_dopri853()
+    print(f"warm-up: {args.warmup} steps to t = {w['t']:.4f}, "
           f"dt_next {w['dt_next']:.4g}", file=sys.stderr, flush=True)
     snap = sim.snapshot()
     forms = {}
@@ -257,7 +275,7 @@ def run_gpu(args, wl):
     sim.drop(snap)
     sim.close()
     return dict(forms=forms, prof=prof, check=check, warm_t=w["t"],
-                copy_gbs=copy_gbs)
+                copy_gbs=copy_gbs, pre_steps=pre["n_ok"] + pre["n_bad"])
 
 
 def ensemble_bench(steps):
@@ -642,9 +660,12 @@ def main():
         "config": {
             "workload": (f"{args.grid}^3 FP64 DOPRI853 rtol 1e-10 atol 1e-14, GW on, "
                          f"filter on, k_min 0.05, m 1, amplitude 1e-5, "
-                         f"TIME_STEP_SKIPS {args.skip}; {args.warmup} warm-up "
-                         f"steps to t = {gpu['warm_t']:.2f}, then {HISTORY_STEPS} "
-                         f"untimed + {args.steps} timed steps per form"),
+                         f"TIME_STEP_SKIPS {args.skip}; untimed pre-roll of "
+                         f"{gpu['pre_steps']} steps from t = 0 to "
+                         f"{PRE_ROLL_T:g} (error-controlled phase), "
+                         f"{args.warmup} warm-up steps to t = "
+                         f"{gpu['warm_t']:.2f}, then {HISTORY_STEPS} untimed + "
+                         f"{args.steps} timed steps per form"),
             "grid": args.grid, "form": "tuned (learned step proposer + success())",
             "tuned": dict(summary(tuned), proposals=tuned["proposals"]),
             "plain": summary(plain),
@@ -658,6 +679,8 @@ def main():
         },
         "roofline": roofline(gpu["prof"], args.grid, gpu["copy_gbs"]),
     }
+    if not args.no_long_run:
+        rec["config"]["long_run"] = long_run(args, wl, LONG_RUN_T)
     if not args.no_cpu_baseline:
         rec["cpu_baseline"] = cpu_baseline(args, wl, gpu["check"], plain["dt"])
     else:

@@ -656,6 +656,11 @@ class Sim:
                                   host_lib().ih_dense_state(self.sim), self.Ntot))
         return out
 
+    def dense_state_active(self):
+        """inside a save() call: whether it saves a dense-output state"""
+        return (host_lib().ih_save_state(self.sim) ==
+                host_lib().ih_dense_state(self.sim))
+
     def read_dat(self, path, with_psi=True):
         """initial data from a .dat file (IC_FROM_DAT_FILE_WITH_PSI, or
         WITHOUT_PSI with with_psi=False)"""

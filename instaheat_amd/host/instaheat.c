@@ -174,15 +174,24 @@ static int do_save(ih_sim *s, ih_save_fn save, void *user, double t)
     return 0;
 }
 
+/* whether a time-based output falls in (t0, t0 + h] */
+static int dense_due(const ih_dp_opts *o, long next, double t0, double h)
+{
+    while (next < o->n_out && o->t_out[next] <= t0) ++next;
+    return next < o->n_out && o->t_out[next] <= t0 + h;
+}
+
 /* DOPRI853 dense output of one accepted step from t0 to t0 + h, called
  * after the swap: field = y_new, dfield = f(t0 + h, y_new), field_new = y,
  * dfield_new = the step's FSAL derivative, stage[1..11] = stages 2..12.
  * For every t_out in (t0, t0 + h]: the three extra stages once (into the
  * arrays of stages 2..4, which the interpolant does not read), the
  * interpolated state into stage[0], an output mk_rhs on it with the
- * summaries reading it, and save(t_out). */
+ * summaries reading it, and save(t_out).  Stage 5's array (stage[4]) is
+ * neither read nor written.  With step_save (a step-based save at t0 + h
+ * follows) a t_out equal to t0 + h is left to that save. */
 static int dense_saves(ih_sim *s, const ih_dp_opts *o, long *next, double t0,
-                       double h, ih_save_fn save, void *user)
+                       double h, int step_save, ih_save_fn save, void *user)
 {
     long i = *next;
     while (i < o->n_out && o->t_out[i] <= t0) ++i;
@@ -204,6 +213,7 @@ static int dense_saves(ih_sim *s, const ih_dp_opts *o, long *next, double t0,
     }
     double *yd = s->stage[0];
     for (; !rc && i < o->n_out && o->t_out[i] <= t0 + h; ++i) {
+        if (step_save && o->t_out[i] == t0 + h) continue;
         double theta = (o->t_out[i] - t0) / h;
         if (theta > 1.0) theta = 1.0;
         rc = ihg_dp853_dense(s->g, theta, h, s->field_new, s->field, k, yd);
@@ -516,9 +526,20 @@ long ih_run_dopri853(ih_sim *s, const ih_dp_opts *opts, ih_save_fn save,
         if (underflow) break;
         s->evo_flags.filter = filter;
         if ((n_stp + 1) % c->skip == 0) s->evo_flags.output = 1;
+        const int output = s->evo_flags.output;
+        /* A step with a step-based save and dense saves: the dense saves'
+         * output calls replace the observables and rho of this output call,
+         * so the call is made once more after them, on a copy of y_new as it
+         * is now (before the filter changes it), in stage 5's array, which
+         * the dense output leaves alone. */
+        const int redo = output && opts->t_out &&
+                         dense_due(opts, next_out, s->t, h);
+        if (redo) {
+            rc = ihg_stream_copy(s->g, s->stage[4], s->field_new, s->Ntot);
+            if (rc) return rc;
+        }
         ihg_set_field(s->g, s->field); /* summaries lag: they read `field` */
         rc = ih_mk_rhs(s, s->t + h, s->field_new, s->dfield_new);
-        const int output = s->evo_flags.output;
         s->evo_flags.output = 0;
         s->evo_flags.filter = 0;
         if (rc) return rc;
@@ -533,7 +554,19 @@ long ih_run_dopri853(ih_sim *s, const ih_dp_opts *opts, ih_save_fn save,
         const double t_old = s->t;
         s->t += h;
         if (opts->t_out) {
-            rc = dense_saves(s, opts, &next_out, t_old, h, save, user);
+            rc = dense_saves(s, opts, &next_out, t_old, h, output, save, user);
+            if (rc) return rc;
+        }
+        if (redo) {
+            /* the same call as above on the same input: its observables,
+             * rho and filtered state are bit for bit the first call's */
+            s->evo_flags.filter = filter;
+            s->evo_flags.output = 1;
+            ihg_set_field(s->g, s->field_new); /* the state before the step */
+            rc = ih_mk_rhs(s, s->t, s->stage[4], s->k_tmp);
+            s->evo_flags.output = 0;
+            s->evo_flags.filter = 0;
+            ihg_set_field(s->g, s->field);
             if (rc) return rc;
         }
         if (h == dt)

@@ -128,3 +128,68 @@ def test_time_based_log_binned_outputs(ih):
         assert o["phi_ps"].sum() > 0
     assert sim.t == pytest.approx(2.0)
     sim.close()
+
+
+def _rho(ih, sim, N):
+    """rho of the simulation's last mk_rhs (ihg_download_strided, stride 1)"""
+    import ctypes
+    out = np.zeros(N)
+    st = (ctypes.c_int * 3)(1, 1, 1)
+    rc = ih.lib().ihg_download_strided(sim.ctx, None, st,
+                                       out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    assert rc == 0, ih.lib().ihg_last_error()
+    return out
+
+
+@pytest.mark.parametrize("gw", [True, False])
+def test_step_saves_unchanged_by_dense_saves(ih, gw):
+    """Steps with a step-based save (skip = 1) and dense saves inside them:
+    the step-based saves carry the observables and rho of the step's own
+    output call, those of the same forced run without time-based outputs
+    (bit for bit but for the spectra's atomic bin sums), and a t_out equal
+    to a step's end is saved once."""
+    shape = (16, 16, 16)
+    N = 16 ** 3
+    y0 = make_state(shape, seed=5, gw=gw, a=1.1)
+    dts = [0.02, 0.03, 0.025]
+    t0 = 0.2
+    ends = list(np.cumsum([t0] + dts)[1:])
+    # interior points of steps 1 and 3, and the end of step 2 (exactly the
+    # sum the stepper forms)
+    t_e2 = (t0 + dts[0]) + dts[1]
+    t_out = [t0 + 0.007, t_e2, t_e2 + 0.01, t_e2 + 0.02]
+
+    def run(with_dense):
+        sim = ih.Sim(shape, bounds=BOX_CONFIG, mass=4.0, gw=gw, bins=16,
+                     method=ih.IH_DOPRI853, ti=t0, tf=10.0, skip=1)
+        sim.set_field(y0)
+        sim.t = t0
+        saves, rhos, dense_flag = [], [], []
+
+        def grab(t):
+            rhos.append(_rho(ih, sim, N))
+            dense_flag.append(sim.dense_state_active())
+        sim.run_dopri853(forced=dts, saves=saves,
+                         t_out=t_out if with_dense else None, on_save=grab)
+        y = sim.field()
+        sim.close()
+        return saves, rhos, dense_flag, y
+
+    s_plain, r_plain, _, y_plain = run(False)
+    s_dense, r_dense, flag, y_dense = run(True)
+    np.testing.assert_array_equal(y_dense, y_plain)
+    step_saves = [(s, r) for s, r, f in zip(s_dense, r_dense, flag) if not f]
+    assert len(step_saves) == len(s_plain) == len(r_plain)
+    for (sd, rd), sp, rp in zip(step_saves, s_plain, r_plain):
+        assert sd[0] == sp[0] and sd[1] == sp[1]
+        # summaries and constraints (fixed-order reductions) bit for bit; the
+        # spectra to the rounding of their atomic bin sums
+        np.testing.assert_array_equal(sd[2][:28], sp[2][:28])
+        np.testing.assert_allclose(sd[2][28:], sp[2][28:], rtol=1e-13, atol=0)
+        np.testing.assert_array_equal(rd, rp)
+    times = [s[0] for s in s_dense]
+    assert times == sorted(times)
+    assert times.count(t_e2) == 1
+    dense_times = [s[0] for s, f in zip(s_dense, flag) if f]
+    assert dense_times == [t for t in t_out if t != t_e2]
+    assert [s[0] for s in s_plain][1:] == pytest.approx(ends, abs=0)
