@@ -129,6 +129,23 @@ int ihg_dp853_stage(ihg_ctx *c, int s, double dt, const double *y,
 int ihg_dp853_finish(ihg_ctx *c, double dt, const double *y,
                      const double *const *k, double *y_new, double rtol,
                      double atol, double sums[2]);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): ihg_dp853_finish without a context.  The state is
+ * given by its sizes: nlive = the live prefix the stage kernels stream
+ * (Ntot - 1 with the GW blocks, 4 N without them; even), ntot = Ntot (the
+ * scale factor is y[ntot - 1]).  The block partials and the two sums live
+ * in a fixed device array of the library (65538 doubles, 512 KiB, one per
+ * GPU, held until the process ends); nothing is allocated or freed.  Work
+ * is queued on `stream` (a hipStream_t of the current device, NULL: the
+ * null stream), which is synchronised before the sums are returned.  Every
+ * call on one GPU shares that GPU's array, so calls that overlap on one GPU
+ * race.  nlive above IHG_NOALLOC_MAX_LIVE is rejected with IHG_EINVAL. */
+#define IHG_NOALLOC_MAX_LIVE 2147483646L
+int ihg_dp853_finish_noalloc(void *stream, long nlive, long ntot, double dt,
+                             const double *y, const double *const *k,
+                             double *y_new, double rtol, double atol,
+                             double sums[2]);
 /* the tableau the stage kernels use (row-major a[13][12], c[13], b[12],
  * bhh[3], er[12]) */
 int ihg_dp853_tableau(double *a, double *c, double *b, double *bhh,

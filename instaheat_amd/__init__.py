@@ -140,6 +140,7 @@ IHG_SYMBOLS = [
     "ihg_sync", "ihg_fft_r2c", "ihg_fft_c2r", "ihg_set_field", "ihg_mk_rhs",
     "ihg_means", "ihg_observables", "ihg_calls_rhs", "ihg_mk_initial_psi",
     "ihg_rk4_stage", "ihg_rk4_final", "ihg_dp853_stage", "ihg_dp853_finish",
+    "ihg_dp853_finish_noalloc",
     "ihg_dp853_tableau", "ihg_dp853_dense_stage", "ihg_dp853_dense",
     "ihg_dp853_dense_tableau", "ihg_gsl_stage", "ihg_gsl_finish",
     "ihg_gsl_tableau", "ihg_download_strided", "ihg_aux", "ihg_prof_enable", "ihg_prof_reset",
@@ -205,6 +206,9 @@ def lib():
         "ihg_dp853_stage": (i, [_vp, i, d, _vp, ctypes.POINTER(_vp), _vp]),
         "ihg_dp853_finish": (i, [_vp, d, _vp, ctypes.POINTER(_vp), _vp, d, d,
                                  _dptr]),
+        "ihg_dp853_finish_noalloc": (i, [_vp, ctypes.c_long, ctypes.c_long, d,
+                                         _vp, ctypes.POINTER(_vp), _vp, d, d,
+                                         _dptr]),
         "ihg_dp853_tableau": (i, [_dptr] * 5),
         "ihg_dp853_dense_stage": (i, [_vp, i, d, _vp, _vp, _vp]),
         "ihg_dp853_dense": (i, [_vp, d, d, _vp, _vp, _vp, _vp]),

@@ -8,6 +8,7 @@
 // writes them), so skipping them changes no value; the error norm still
 // divides by Ntot.
 #include <cmath>
+#include <mutex>
 
 #include "dp853_tableau.h"
 #include "gsl_tableau.h"
@@ -543,27 +544,44 @@ extern "C" int ihg_dp853_dense_tableau(double *c_extra, double *a_extra,
     return 0;
 }
 
-extern "C" int ihg_dp853_finish(ihg_ctx *c, double dt, const double *y,
-                                const double *const *k, double *y_new,
-                                double rtol, double atol, double sums[2])
+// the stage arrays the final sum reads: stages 2..5 carry no b / er weight,
+// so their slots repeat k[0] (never dereferenced for a weight of 0)
+static int finish_inputs(const double *y, const double *const *k,
+                         const double *y_new, FinishIn &in)
 {
-    if (!c || !k || !sums) return set_error("NULL argument");
+    if (!k) return set_error("NULL argument");
     IHG_TRY(check_ptr(y, "y"));
     IHG_TRY(check_ptr(y_new, "y_new"));
-    FinishIn in{};
     for (int j = 0; j < 12; ++j) {
-        if (j >= 1 && j <= 4) continue; // stages 2..5 carry no b / er weight
+        if (j >= 1 && j <= 4) continue;
         IHG_TRY(check_ptr(k[j], "k[j]"));
         in.k[j] = k[j];
     }
     for (int j = 1; j <= 4; ++j)
         in.k[j] = k[0];
-    const long nl = live_len(c);
-    // about four pairs per thread: enough work-groups in flight to keep the
-    // nine input streams at full rate (the block partials stay small)
+    return 0;
+}
+
+// work-groups of k_dp_finish: about four pairs per thread, enough
+// work-groups in flight to keep the nine input streams at full rate, at
+// most FINISH_MAX_GROUPS (the kernel strides over the rest)
+constexpr long FINISH_MAX_GROUPS = 32768;
+static long finish_groups(long nl)
+{
     long nb = (nl / 2 + RED_WG * 4 - 1) / (RED_WG * 4);
-    if (nb > 32768) nb = 32768;
-    if (nb < 1) nb = 1;
+    if (nb > FINISH_MAX_GROUPS) nb = FINISH_MAX_GROUPS;
+    return nb < 1 ? 1 : nb;
+}
+
+extern "C" int ihg_dp853_finish(ihg_ctx *c, double dt, const double *y,
+                                const double *const *k, double *y_new,
+                                double rtol, double atol, double sums[2])
+{
+    if (!c || !sums) return set_error("NULL argument");
+    FinishIn in{};
+    IHG_TRY(finish_inputs(y, k, y_new, in));
+    const long nl = live_len(c);
+    const long nb = finish_groups(nl);
     if (2 * nb + 2 > c->npartials) return set_error("partials buffer too small");
     // y and the 8 weighted stages in, y_new out
     IHG_TRY(timed(c, F_DP_FINISH, 10.0 * 8.0 * nl, [&]() {
@@ -580,6 +598,70 @@ extern "C" int ihg_dp853_finish(ihg_ctx *c, double dt, const double *y,
     IHG_HIP(hipStreamSynchronize(c->stream));
     sums[0] = c->hscal[8];
     sums[1] = c->hscal[9];
+    return 0;
+}
+
+// ---- ihg_dp853_finish without a context and without allocation --------
+// The block partials and the two sums live in `scratch`, one copy per GPU
+// for the life of the process.  Host code never passes `scratch` itself:
+// its device address, looked up once per device, is kept in scratch_dev.
+constexpr long NOALLOC_SUMS = 2 * FINISH_MAX_GROUPS; // offset of the sums
+constexpr long NOALLOC_DOUBLES = NOALLOC_SUMS + 2;
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static double *scratch_dev[NOALLOC_MAX_DEVICES];
+static std::mutex scratch_mu;
+
+static int scratch_of_current_device(double **out)
+{
+    int dev = -1;
+    IHG_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+        return set_error("device %d beyond the %d devices the scratch table "
+                         "holds", dev, NOALLOC_MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_dev[dev]) {
+        void *p = nullptr;
+        IHG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return set_error("hipGetSymbolAddress gave no address");
+        scratch_dev[dev] = static_cast<double *>(p);
+    }
+    *out = scratch_dev[dev];
+    return 0;
+}
+
+extern "C" int ihg_dp853_finish_noalloc(void *stream, long nlive, long ntot,
+                                        double dt, const double *y,
+                                        const double *const *k, double *y_new,
+                                        double rtol, double atol,
+                                        double sums[2])
+{
+    if (!sums) return set_error("NULL argument");
+    if (nlive < 2 || (nlive & 1) || ntot < nlive + 1)
+        return set_error("nlive = %ld must be even, at least 2 and below "
+                         "ntot = %ld", nlive, ntot);
+    if (nlive > IHG_NOALLOC_MAX_LIVE)
+        return set_error("nlive = %ld beyond the maximum %ld of "
+                         "ihg_dp853_finish_noalloc", nlive,
+                         (long)IHG_NOALLOC_MAX_LIVE);
+    FinishIn in{};
+    IHG_TRY(finish_inputs(y, k, y_new, in));
+    double *part = nullptr;
+    IHG_TRY(scratch_of_current_device(&part));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const long nb = finish_groups(nlive);
+    // every partial k_sum2_final reads is written by k_dp_finish first, so
+    // the scratch needs no clearing between calls
+    hipLaunchKernelGGL(k_dp_finish, dim3((unsigned)nb), dim3(RED_WG), 0, s,
+                       y, in, y_new, dt, rtol, atol, nlive, ntot, part);
+    IHG_LAUNCHED("k_dp_finish");
+    hipLaunchKernelGGL(k_sum2_final, dim3(1), dim3(RED_WG), 0, s, part,
+                       (int)nb, part + NOALLOC_SUMS);
+    IHG_LAUNCHED("k_sum2_final");
+    IHG_HIP(hipMemcpyAsync(sums, part + NOALLOC_SUMS, 2 * sizeof(double),
+                           hipMemcpyDeviceToHost, s));
+    IHG_HIP(hipStreamSynchronize(s));
     return 0;
 }
 
