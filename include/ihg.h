@@ -107,6 +107,19 @@ int ihg_observables(ihg_ctx *c, double *out, size_t len);
 /* mon.calls_rhs (src/toolbox.c:77) */
 unsigned long ihg_calls_rhs(const ihg_ctx *c);
 
+/* mean (src/toolbox.c:726-734) and variance (:744-754, given the mean:
+ * (sum d^2 - (sum d)^2 / n) / (n - 1), d = x - mean) of n doubles of device
+ * memory, with deterministic reductions.  Synchronise. */
+int ihg_mean(ihg_ctx *c, const double *x, size_t n, double *out);
+int ihg_variance(ihg_ctx *c, double mean, const double *x, size_t n,
+                 double *out);
+/* out[0] = sum (a - b)^2, out[1] = sum (cc - d)^2 over n doubles of device
+ * memory: the two sums of check_for_stiffness (src/dopri853.c:550-560).
+ * Synchronises. */
+int ihg_diff_norms(ihg_ctx *c, const double *a, const double *b,
+                   const double *cc, const double *d, size_t n,
+                   double out[2]);
+
 /* mk_initial_psi (src/setup.c:825-903): psi, dpsi from phi, dphi, a */
 int ihg_mk_initial_psi(ihg_ctx *c, double ti, double *field, double *dfield);
 

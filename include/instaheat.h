@@ -51,6 +51,15 @@ typedef struct ih_config {
     double max_dt_hubble_fraction; /* MAX_DT_HUBBLE_FRACTION (<= 0: off) */
     long skip;                /* TIME_STEP_SKIPS */
     double a_initial;         /* A_INITIAL */
+    double max_runtime;       /* MAX_RUNTIME: run_dopri853 stops after a
+                                 step once this many seconds have passed
+                                 since ih_create (src/dopri853.c:145-149);
+                                 <= 0: no limit */
+    int enable_followup;      /* ENABLE_FOLLOWUP: run_dopri853's wrap-up
+                                 calls ih_dp_opts.followup
+                                 (src/dopri853.c:229-231) */
+    int enable_stiffness_check; /* ENABLE_STIFFNESSCHECK
+                                   (src/dopri853.c:295-299, 550-580) */
 } ih_config;
 
 /* parameters.sh defaults (src/parameters.sh) for the given grid */
@@ -75,6 +84,9 @@ struct ih_evolution_flags *ih_flags(ih_sim *s);
 double ih_time(const ih_sim *s);
 /* sets t and clears the step history the proposer reads */
 void ih_set_time(ih_sim *s, double t);
+/* pars.t.ti: INITIAL_TIME of the next run (the GSL driver's output times,
+ * src/gsl_stepper.c:60); the HDF5 readers set it with the time */
+void ih_set_ti(ih_sim *s, double ti);
 /* pars.t.dt: the first step size of the next run (DELTA_T initially) */
 void ih_set_dt(ih_sim *s, double dt);
 /* MAX_STEPS of the next run */
@@ -100,6 +112,10 @@ int ih_read_dat(ih_sim *s, const char *path, int with_psi);
 /* the current phi, dphi, psi, dpsi in that format (no reference writer) */
 int ih_write_dat(ih_sim *s, const char *path);
 
+/* prepare_and_save_timeslice (src/toolbox.c:96-104): a filtered output
+ * mk_rhs on field into dfield, then save() */
+int ih_prepare_and_save_timeslice(ih_sim *s, ih_save_fn save, void *user);
+
 /* run_rk4 (src/rk4.c:27-132); returns the number of steps or < 0 */
 long ih_run_rk4(ih_sim *s, ih_save_fn save, void *user);
 
@@ -112,12 +128,37 @@ typedef struct ih_dp_record {
     int accepted;
 } ih_dp_record;
 
+/* why run_dopri853 stopped (the messages of src/dopri853.c:136-153, 280) */
+#define IH_STOP_FINAL_TIME 1   /* "Reached specified final time." */
+#define IH_STOP_MAX_STEPS 2    /* "Reached maximal number of steps." */
+#define IH_STOP_MAX_RUNTIME 3  /* "Reached specified maximal runtime." */
+#define IH_STOP_DT_MIN 4       /* "Stepsize underflow." (dt_next <= dt_min) */
+#define IH_STOP_UNDERFLOW 5    /* perform_step's underflow (|dt| <= |t| eps) */
+#define IH_STOP_FORCED 6       /* the forced step sequence is done */
+#define IH_STOP_STIFF 7        /* "Potential stiffness detected" */
+/* run_dopri853's return value when the stiffness check fires (the
+ * reference exits the process there, without the wrap-up) */
+#define IH_ESTIFF (-2001)
+
 typedef struct ih_dp_stats {
     long n_stp, n_ok, n_bad, attempts;
     double t, dt_next, err_old;
     double integration_seconds; /* mon.integration (src/dopri853.c:117,156) */
     unsigned long calls_rhs;
+    int stop;                   /* IH_STOP_* */
+    double dtlamb;              /* dp.dtlamb of the stiffness check */
 } ih_dp_stats;
+
+/* The stiffness check's counters (check_for_stiffness, src/dopri853.c:
+ * 550-580): ih_stiffness_update takes the two sums of ihg_diff_norms (num =
+ * |dfield_new - k12|^2, den = |field_new - stage 12 input|^2) and the step's
+ * first trial dt, and returns 1 on the 15th stiff verdict in a row (at most
+ * 5 non-stiff ones between them). */
+typedef struct ih_stiffness {
+    double dtlamb;
+    int stiff, nonstiff;
+} ih_stiffness;
+int ih_stiffness_update(ih_stiffness *st, double dt, double num, double den);
 
 /* Optional step proposer, consulted after success() has accepted a step
  * that did not follow a rejection: history holds the last n_history <= 8
@@ -141,6 +182,16 @@ typedef struct ih_dp_opts {
      * step-based outputs every `skip` steps are kept); t_out increasing */
     const double *t_out;
     long n_out;
+    /* The output file (src/io.c:494-613, 822-827), both may be NULL:
+     * buffered() = slices saved but not yet written (the reference's
+     * pars.file.index); with it the wrap-up saves the final slice only when
+     * buffered() != 0 and the last save is older than t, as
+     * src/dopri853.c:223-226 do; without it, whenever the last save is
+     * older than t.  followup() writes the followup state (h5_write_followup)
+     * in the wrap-up when enable_followup is set. */
+    long (*buffered)(void *file);
+    int (*followup)(void *file);
+    void *file;
 } ih_dp_opts;
 
 /* the state of the latest dense output (device, Ntot doubles); valid

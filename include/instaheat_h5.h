@@ -171,6 +171,11 @@ int ih_h5_write_dp_stats(ih_h5 *h, const ih_dp_stats *st);
 /* h5_write_followup (src/io.c:822-827): the bound simulation's state
  * (Ntot doubles, contiguous layout) */
 int ih_h5_write_followup(ih_h5 *h);
+/* ih_dp_opts.buffered / .followup for run_dopri853's wrap-up
+ * (src/dopri853.c:223-231), file = the ih_h5: slices saved and not yet
+ * written (pars.file.index, src/io.c:604-608), and ih_h5_write_followup */
+long ih_h5_buffered(void *h5);
+int ih_h5_followup(void *h5);
 /* h5_close (src/io.c:472-488): write the partial buffer and, as main does
  * at the end of every run (src/main.c:99-106), runtime_total (wall
  * seconds since ih_h5_create) and calls_rhs (the bound simulation's count
@@ -181,14 +186,14 @@ int ih_h5_close(ih_h5 *h);
 const char *ih_h5_last_error(void);
 
 /* h5_read_followup (src/io.c:761-801), the IC_FROM_H5_FILE initial data:
- * the state from `followup` (Ntot doubles) and the simulation's time set
- * to the file's last time */
+ * the state from `followup` (Ntot doubles) and the simulation's time and
+ * INITIAL_TIME (ih_set_ti) set to the file's last time */
 int ih_h5_read_followup(ih_sim *s, const char *path);
 /* h5_read_timeslice (src/io.c:640-711): the first time slice with
  * time + DBL_EPSILON >= t (else the last), phi, dphi, psi, dpsi and a from
  * it (the full-field datasets at stride 1), the other state entries zero;
- * the simulation's time is set to the slice's time (also in *t_found when
- * not NULL) */
+ * the simulation's time and INITIAL_TIME (ih_set_ti) are set to the
+ * slice's time (also in *t_found when not NULL) */
 int ih_h5_read_timeslice(ih_sim *s, const char *path, double t,
                          double *t_found);
 /* Read a double dataset: dims[0], dims[1] its extent (dims[1] = 1 for

@@ -69,6 +69,8 @@ class IhConfig(ctypes.Structure):
         ("rtol", ctypes.c_double), ("atol", ctypes.c_double),
         ("max_dt_hubble_fraction", ctypes.c_double),
         ("skip", ctypes.c_long), ("a_initial", ctypes.c_double),
+        ("max_runtime", ctypes.c_double), ("enable_followup", ctypes.c_int),
+        ("enable_stiffness_check", ctypes.c_int),
     ]
 
 
@@ -92,7 +94,23 @@ class IhDpStats(ctypes.Structure):
                 ("t", ctypes.c_double), ("dt_next", ctypes.c_double),
                 ("err_old", ctypes.c_double),
                 ("integration_seconds", ctypes.c_double),
-                ("calls_rhs", ctypes.c_ulong)]
+                ("calls_rhs", ctypes.c_ulong), ("stop", ctypes.c_int),
+                ("dtlamb", ctypes.c_double)]
+
+
+# why run_dopri853 stopped (ih_dp_stats.stop)
+IH_STOP_FINAL_TIME, IH_STOP_MAX_STEPS, IH_STOP_MAX_RUNTIME, IH_STOP_DT_MIN, \
+    IH_STOP_UNDERFLOW, IH_STOP_FORCED, IH_STOP_STIFF = range(1, 8)
+IH_ESTIFF = -2001
+
+
+class IhStiffness(ctypes.Structure):
+    _fields_ = [("dtlamb", ctypes.c_double), ("stiff", ctypes.c_int),
+                ("nonstiff", ctypes.c_int)]
+
+
+BUFFERED_FN = ctypes.CFUNCTYPE(ctypes.c_long, _vp)
+FOLLOWUP_FN = ctypes.CFUNCTYPE(ctypes.c_int, _vp)
 
 
 PROPOSER_FN = ctypes.CFUNCTYPE(ctypes.c_double, _vp,
@@ -112,7 +130,9 @@ class IhDpOpts(ctypes.Structure):
     _fields_ = [("forced_dt", _dptr), ("n_forced", ctypes.c_long),
                 ("log", ctypes.POINTER(IhDpRecord)), ("log_cap", ctypes.c_long),
                 ("proposer", PROPOSER_FN), ("proposer_user", _vp),
-                ("t_out", _dptr), ("n_out", ctypes.c_long)]
+                ("t_out", _dptr), ("n_out", ctypes.c_long),
+                ("buffered", BUFFERED_FN), ("followup", FOLLOWUP_FN),
+                ("file", _vp)]
 
 
 class IhMlpProposer(ctypes.Structure):
@@ -140,7 +160,7 @@ IHG_SYMBOLS = [
     "ihg_sync", "ihg_fft_r2c", "ihg_fft_c2r", "ihg_set_field", "ihg_mk_rhs",
     "ihg_means", "ihg_observables", "ihg_calls_rhs", "ihg_mk_initial_psi",
     "ihg_rk4_stage", "ihg_rk4_final", "ihg_dp853_stage", "ihg_dp853_finish",
-    "ihg_dp853_finish_noalloc",
+    "ihg_dp853_finish_noalloc", "ihg_mean", "ihg_variance", "ihg_diff_norms",
     "ihg_dp853_tableau", "ihg_dp853_dense_stage", "ihg_dp853_dense",
     "ihg_dp853_dense_tableau", "ihg_gsl_stage", "ihg_gsl_finish",
     "ihg_gsl_tableau", "ihg_download_strided", "ihg_aux", "ihg_prof_enable", "ihg_prof_reset",
@@ -156,7 +176,8 @@ IHG_SYMBOLS = [
 HOST_SYMBOLS = [
     "ih_config_defaults", "ih_create", "ih_destroy", "ih_gpu", "ih_cfg",
     "ih_field", "ih_dfield", "ih_flags", "ih_time", "ih_set_time", "ih_set_dt",
-    "ih_set_max_steps", "ih_set_tf",
+    "ih_set_max_steps", "ih_set_tf", "ih_set_ti",
+    "ih_prepare_and_save_timeslice", "ih_stiffness_update",
     "ih_mk_rhs", "ih_init_gaussian", "ih_mk_initial_psi", "ih_read_dat",
     "ih_write_dat", "ih_run_rk4",
     "ih_run_dopri853", "ih_dense_state", "ih_save_state", "ih_success", "ih_error_norm", "ih_proposer_features",
@@ -209,6 +230,9 @@ def lib():
         "ihg_dp853_finish_noalloc": (i, [_vp, ctypes.c_long, ctypes.c_long, d,
                                          _vp, ctypes.POINTER(_vp), _vp, d, d,
                                          _dptr]),
+        "ihg_mean": (i, [_vp, _vp, sz, _dptr]),
+        "ihg_variance": (i, [_vp, d, _vp, sz, _dptr]),
+        "ihg_diff_norms": (i, [_vp, _vp, _vp, _vp, _vp, sz, _dptr]),
         "ihg_dp853_tableau": (i, [_dptr] * 5),
         "ihg_dp853_dense_stage": (i, [_vp, i, d, _vp, _vp, _vp]),
         "ihg_dp853_dense": (i, [_vp, d, d, _vp, _vp, _vp, _vp]),
@@ -284,6 +308,9 @@ def host_lib():
         "ih_set_dt": (None, [_vp, d]),
         "ih_set_max_steps": (None, [_vp, ctypes.c_long]),
         "ih_set_tf": (None, [_vp, ctypes.c_double]),
+        "ih_set_ti": (None, [_vp, ctypes.c_double]),
+        "ih_prepare_and_save_timeslice": (i, [_vp, SAVE_FN, _vp]),
+        "ih_stiffness_update": (i, [ctypes.POINTER(IhStiffness), d, d, d]),
         "ih_cfg": (_vp, [_vp]),
         "ih_flags": (_vp, [_vp]),
         "ih_mk_rhs": (i, [_vp, d, _vp, _vp]),
@@ -558,7 +585,8 @@ class Sim:
                  dt=1e-7, dt_min=1e-9, max_steps=10**15, rtol=1e-10,
                  atol=1e-14, hubble_frac=1e-2, skip=1, a_initial=1.0,
                  beta=0.0, safe=0.9, minscale=0.333, maxscale=3.0,
-                 gw_fp32=False, log_bins=False):
+                 gw_fp32=False, log_bins=False, max_runtime=None,
+                 followup=False, stiffness_check=False):
         cfg = default_config()
         cfg.grid = make_params(shape, bounds, mass, gw, filt, bins, gw_fp32,
                                log_bins)
@@ -568,6 +596,10 @@ class Sim:
         cfg.beta, cfg.safe, cfg.minscale, cfg.maxscale = beta, safe, minscale, maxscale
         cfg.rtol, cfg.atol, cfg.max_dt_hubble_fraction = rtol, atol, hubble_frac
         cfg.skip, cfg.a_initial = skip, a_initial
+        if max_runtime is not None:
+            cfg.max_runtime = max_runtime
+        cfg.enable_followup = int(bool(followup))
+        cfg.enable_stiffness_check = int(bool(stiffness_check))
         self.cfg = cfg
         self.bins = bins
         s = _vp()
@@ -594,6 +626,23 @@ class Sim:
     @t.setter
     def t(self, v):
         host_lib().ih_set_time(self.sim, v)
+
+    def set_ti(self, ti):
+        """INITIAL_TIME of the next run (the GSL driver's output times)"""
+        host_lib().ih_set_ti(self.sim, ti)
+
+    def prepare_and_save_timeslice(self, on_save=None):
+        """prepare_and_save_timeslice: a filtered output RHS on the state and
+        a save; returns (t, a, observables)"""
+        got = []
+
+        def cb(user, t, a, obs, n):
+            got.append((t, a, np.ctypeslib.as_array(obs, (n,)).copy()))
+            if on_save is not None:
+                on_save(t)
+        fn = SAVE_FN(cb)
+        _check(host_lib().ih_prepare_and_save_timeslice(self.sim, fn, None))
+        return got[0]
 
     def set_dt(self, dt):
         host_lib().ih_set_dt(self.sim, dt)
@@ -732,12 +781,16 @@ class Sim:
             prop, user = proposer.c_proposer()
         else:
             prop = PROPOSER_FN(lambda u, h, n, dtc: proposer(h, n, dtc))
+        buffered, followup, file = BUFFERED_FN(), FOLLOWUP_FN(), None
+        if h5 is not None:
+            buffered, followup, file = h5.wrapup_hooks()
         opts = IhDpOpts(fz.ctypes.data_as(_dptr) if fz is not None else None,
                         len(fz) if fz is not None else 0,
                         ctypes.cast(log, ctypes.POINTER(IhDpRecord)) if log else None,
                         log_cap, prop, user,
                         to.ctypes.data_as(_dptr) if to is not None else None,
-                        len(to) if to is not None else 0)
+                        len(to) if to is not None else 0,
+                        buffered, followup, file)
         st = IhDpStats()
 
         def cb(user, t, a, obs, n):
@@ -750,6 +803,9 @@ class Sim:
         fn = SAVE_FN(cb)
         n = host_lib().ih_run_dopri853(self.sim, ctypes.byref(opts), fn, None,
                                        ctypes.byref(st))
+        if n == IH_ESTIFF:
+            raise IhgError(f"potential stiffness detected at t = {st.t} "
+                           f"(dtlamb {st.dtlamb})")
         if n < 0:
             _check(n)
         out = {k: getattr(st, k) for k, _ in IhDpStats._fields_}

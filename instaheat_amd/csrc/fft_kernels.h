@@ -12,6 +12,7 @@
 #pragma once
 
 #include "fft_core.h"
+#include "mem.h"
 
 namespace ihg {
 
@@ -86,11 +87,11 @@ __global__ __launch_bounds__(WG) void k_c2c_strided(
     C *dst = out + base;
     const long st = ls.stride;
     auto rd = [&](int, int n) {
-        const C v = src[n * st]; // in bounds for every lane
+        const C v = ldn(src + n * st); // in bounds for every lane
         return act ? v : CT<C>::make(0, 0);
     };
     auto wr = [&](int, int n, C v) {
-        if (act) dst[n * st] = v;
+        if (act) stn(dst + n * st, v);
     };
     fft_line<L, DIR, true, true, P>(lds + b * PITCH, j, tw, rd, wr);
 }
@@ -188,7 +189,7 @@ __global__ __launch_bounds__(WG) void k_x_fanout(
     double2 *my = lds + b * PITCH;
     double2 spec[E];
     auto rd = [&](int, int n) {
-        const double2 v = src[n * st]; // in bounds for every lane
+        const double2 v = ldn(src + n * st); // in bounds for every lane
         return act ? v : make_double2(0.0, 0.0);
     };
     if constexpr (FWD) {
@@ -208,7 +209,7 @@ __global__ __launch_bounds__(WG) void k_x_fanout(
             return fac(q, line, n, spec[FWD ? chain_slot<L, P>(slot) : slot]);
         };
         auto wq = [&](int, int n, double2 v) {
-            if (act) dst[n * st] = v;
+            if (act) stn(dst + n * st, v);
         };
         fft_line<L, +1, true, true, P>(my, j, tw, rq, wq);
     }
@@ -279,7 +280,7 @@ __global__ __launch_bounds__(FFT_WG) void k_z_r2c(
     const bool act = line < nlines;
     const double *ra = in + (act ? 2 * line : 0) * L;
     auto rd = [&](int, int n) {
-        const double u = ra[n], v = ra[L + n]; // in bounds for every lane
+        const double u = ldn(ra + n), v = ldn(ra + L + n); // in bounds
         return act ? make_double2(u, v) : make_double2(0.0, 0.0);
     };
     fft_line<L, -1, true>(lds + bc * TL::PITCH, tid % TL::T, tw, rd);
@@ -287,8 +288,8 @@ __global__ __launch_bounds__(FFT_WG) void k_z_r2c(
     for (int idx = tid; idx < 2 * TL::B * MZ; idx += FFT_WG) {
         const int r = idx / MZ, k = idx - r * MZ;
         if (row0 + r < nrows)
-            out[row0 * MZ + idx] =
-                herm_split<L>(lds + (r >> 1) * TL::PITCH, k, r & 1);
+            stn(out + row0 * MZ + idx,
+                herm_split<L>(lds + (r >> 1) * TL::PITCH, k, r & 1));
     }
 }
 
@@ -319,7 +320,7 @@ __global__ __launch_bounds__(FFT_WG) void k_z_c2r(
         const int r = idx / L, n = idx - r * L;
         if (row0 + r < nrows) {
             const double2 z = lds[(r >> 1) * TL::PITCH + Lds<L>::idx(n)];
-            out[row0 * L + idx] = (r & 1) ? z.y : z.x;
+            stn(out + row0 * L + idx, (r & 1) ? z.y : z.x);
         }
     }
 }

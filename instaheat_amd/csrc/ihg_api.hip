@@ -9,6 +9,7 @@
 #include <cstdint>
 
 #include "ihg_internal.h"
+#include "mem.h"
 
 namespace ihg {
 
@@ -67,9 +68,14 @@ static int upload_now(void *dst, const void *src, size_t bytes,
     return hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
 }
 
-// filter_window / mk_filter_mask (src/setup.c:710-754), evaluated on the
-// host with the same libm calls as the reference
-static void host_filter_mask(const ihg_ctx *c, double *out)
+// filter_window / mk_filter_mask (src/setup.c:710-754): the window
+// exp(-36 x^36) of x = max_i |k_i| / k_i^max.  It decreases with x, so the
+// window of the largest axis fraction is the smallest of the three axis
+// windows: F(i, j, k) = min(Fx[i], Fy[j], Fz[k]) exactly.  The three axis
+// tables (Nx + Ny + Mz values) are evaluated on the host with the same libm
+// calls as the reference; the kernels take the minimum per point instead of
+// reading an M-value mask.
+static void host_filter_axes(const ihg_ctx *c, double *out)
 {
     const int Nx = c->Nx, Ny = c->Ny, Nz = c->Nz, Mz = c->Mz;
     const KGeom &kg = c->kg;
@@ -78,17 +84,15 @@ static void host_filter_mask(const ihg_ctx *c, double *out)
     const double kzmax = (Nz / 2) * kg.kz1;
     for (int i = 0; i < Nx; ++i) {
         const double kxf = (i > Nx / 2) ? kg.kx1 * (i - Nx) : kg.kx1 * i;
-        for (int j = 0; j < Ny; ++j) {
-            const double kyf = (j > Ny / 2) ? kg.ky1 * (j - Ny) : kg.ky1 * j;
-            for (int k = 0; k < Mz; ++k) {
-                const double kzf = kg.kz1 * k;
-                double frac = fabs(kxf) / kxmax;
-                const double fy = fabs(kyf) / kymax, fz = fabs(kzf) / kzmax;
-                frac = frac > fy ? frac : fy;
-                frac = frac > fz ? frac : fz;
-                out[((long)i * Ny + j) * Mz + k] = exp(-36.0 * pow(frac, 36));
-            }
-        }
+        out[i] = exp(-36.0 * pow(fabs(kxf) / kxmax, 36));
+    }
+    for (int j = 0; j < Ny; ++j) {
+        const double kyf = (j > Ny / 2) ? kg.ky1 * (j - Ny) : kg.ky1 * j;
+        out[Nx + j] = exp(-36.0 * pow(fabs(kyf) / kymax, 36));
+    }
+    for (int k = 0; k < Mz; ++k) {
+        const double kzf = kg.kz1 * k;
+        out[Nx + Ny + k] = exp(-36.0 * pow(fabs(kzf) / kzmax, 36));
     }
 }
 
@@ -294,10 +298,12 @@ static int create_ctx(const ihg_params *p, int nreal, ihg_ctx **out)
         if (rc) return fail(rc);
     }
     if (p->enable_filter) {
-        double *h = (double *)malloc(c->M * sizeof(double));
-        host_filter_mask(c, h);
-        if ((rc = dalloc(&c->filter, c->M, c->stream)) == 0)
-            rc = upload_now(c->filter, h, c->M * sizeof(double), c->stream);
+        const long nf = (long)c->Nx + c->Ny + c->Mz;
+        double *h = (double *)malloc(nf * sizeof(double));
+        if (!h) return fail(set_error("out of host memory"));
+        host_filter_axes(c, h);
+        if ((rc = dalloc(&c->filter, nf, c->stream)) == 0)
+            rc = upload_now(c->filter, h, nf * sizeof(double), c->stream);
         free(h);
         if (rc) return fail(rc);
     }
@@ -416,14 +422,15 @@ extern "C" int ihg_copy(ihg_ctx *c, double *dst, const double *src,
 // a plain copy kernel (one double2 per thread; an odd last element by
 // thread 0 of block 0): state snapshots, and the HBM ceiling and counter
 // calibration the pass figures are compared with
+template <bool NT>
 __global__ __launch_bounds__(256) void k_stream_copy(
     const double *__restrict__ src, double *__restrict__ dst, long count)
 {
     const long n2 = count / 2;
     const long i = (long)blockIdx.x * 256 + threadIdx.x;
     if (i < n2)
-        reinterpret_cast<double2 *>(dst)[i] =
-            reinterpret_cast<const double2 *>(src)[i];
+        ihg::st2<NT>(reinterpret_cast<double2 *>(dst) + i,
+                     ihg::ld2<NT>(reinterpret_cast<const double2 *>(src) + i));
     if (i == 0 && (count & 1))
         dst[count - 1] = src[count - 1];
 }
@@ -437,8 +444,10 @@ extern "C" int ihg_stream_copy(ihg_ctx *c, double *dst, const double *src,
     if (count == 0) return 0;
     const long n2 = (long)(count / 2) > 0 ? (long)(count / 2) : 1;
     return timed(c, F_STREAM_COPY, 16.0 * count, [&]() {
-        hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)((n2 + 255) / 256)),
-                           dim3(256), 0, c->stream, src, dst, (long)count);
+        const bool nt = 8.0 * (double)count >= ihg::NT_BYTES;
+        hipLaunchKernelGGL((nt ? k_stream_copy<true> : k_stream_copy<false>),
+                           dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0,
+                           c->stream, src, dst, (long)count);
         return hip_check(hipGetLastError(), "k_stream_copy");
     });
 }

@@ -96,7 +96,8 @@ struct ihg_ctx {
     hipStream_t stream;
     double2 *tw;       // twiddle tables, tw[L + m] = e^{-2 pi i m / L}
     float2 *twf;       // the same rounded to FP32 (FP32 GW source path)
-    double *filter;    // filter mask, M values (filter in the reference)
+    double *filter;    // filter window per axis: Fx[Nx] | Fy[Ny] | Fz[Mz]
+                       // (the mask `filter` of the reference is their min)
     double2 *W;        // 6 spectral work arrays (M complex each)
     double2 *Wc;       // 2 more spectral arrays (phic / psic for output)
     double *grad3;     // d_x phi, d_y phi, d_z phi (N each) for GW sources
@@ -157,6 +158,7 @@ enum {
     S_DPHI_MEAN = 3, // mean of f[N..2N)
     S_DDPHI_MEAN = 4,
     S_FEATURES = 8, // a, variance of phi, mean of psi^2 (step proposer)
+    S_STAT = 12,    // mean and variance of ihg_mean / ihg_variance
     S_NSCAL = 16
 };
 

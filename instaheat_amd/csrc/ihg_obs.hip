@@ -524,6 +524,47 @@ int features(ihg_ctx *c, const double *f, double out[3])
     return 0;
 }
 
+__global__ void k_set_scalar(double *__restrict__ p, double v) { *p = v; }
+
+// mean / variance of one device array with the summaries' reductions
+// (mean and variance, src/toolbox.c:726-754); the results land in the
+// scalar slots S_STAT (mean) and S_STAT + 1 (variance)
+static int array_stat(ihg_ctx *c, const double *d, long n, int var,
+                      double mean, double *out)
+{
+    const unsigned nb = red_blocks(n);
+    Ptrs8 a8{{d, d, d, d, d, d, d, d}};
+    double *slot = c->dscal + S_STAT;
+    if (!var) {
+        hipLaunchKernelGGL(k_stats1, dim3(nb, 1), dim3(RED_WG), 0, c->stream,
+                           a8, n, c->partials);
+        IHG_LAUNCHED("k_stats1");
+        Dst8 dst{};
+        dst.mean[0] = slot;
+        hipLaunchKernelGGL(k_stats1_final, dim3(1), dim3(RED_WG), 0,
+                           c->stream, c->partials, (int)nb, n, dst);
+        IHG_LAUNCHED("k_stats1_final");
+    } else {
+        // k_stats2 reads the mean from slot[0] and writes slot[1]
+        hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, c->stream, slot,
+                           mean);
+        IHG_LAUNCHED("k_set_scalar");
+        hipLaunchKernelGGL(k_stats2, dim3(nb, 1), dim3(RED_WG), 0, c->stream,
+                           a8, n, slot, c->partials);
+        IHG_LAUNCHED("k_stats2");
+        hipLaunchKernelGGL(k_stats2_final, dim3(1), dim3(RED_WG), 0,
+                           c->stream, c->partials, (int)nb, n, slot);
+        IHG_LAUNCHED("k_stats2_final");
+    }
+    IHG_TRY(hip_check(hipMemcpyAsync(c->hscal + S_STAT, slot,
+                                     2 * sizeof(double),
+                                     hipMemcpyDeviceToHost, c->stream),
+                      "download statistic"));
+    IHG_TRY(hip_check(hipStreamSynchronize(c->stream), "sync"));
+    *out = c->hscal[S_STAT + (var ? 1 : 0)];
+    return 0;
+}
+
 int initial_psi(ihg_ctx *c, double *field, double *dfield)
 {
     const long N = c->N, M = c->M;
@@ -558,3 +599,18 @@ int initial_psi(ihg_ctx *c, double *field, double *dfield)
 }
 
 } // namespace ihg
+
+extern "C" int ihg_mean(ihg_ctx *c, const double *d, size_t n, double *out)
+{
+    if (!c || !d || !out) return ihg::set_error("NULL argument");
+    if (n < 1) return ihg::set_error("mean of an empty array");
+    return ihg::array_stat(c, d, (long)n, 0, 0.0, out);
+}
+
+extern "C" int ihg_variance(ihg_ctx *c, double mean, const double *d,
+                            size_t n, double *out)
+{
+    if (!c || !d || !out) return ihg::set_error("NULL argument");
+    if (n < 2) return ihg::set_error("variance of fewer than two values");
+    return ihg::array_stat(c, d, (long)n, 1, mean, out);
+}

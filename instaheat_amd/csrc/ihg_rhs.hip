@@ -22,6 +22,7 @@
 #include "dispatch.h"
 #include "fft_kernels.h"
 #include "ihg_internal.h"
+#include "mem.h"
 #include "reduce.h"
 
 namespace ihg {
@@ -60,16 +61,20 @@ struct DerivFac {
 // (src/toolbox.c:164-171): phi's in keep_phi (and keep_phi2), psi's in
 // keep_psi; null pointers keep nothing.
 struct FilterFac {
-    const double *__restrict__ filt;
+    const double *__restrict__ filt; // Fx[Nx] | Fy[Ny] | Fz[Mz]
     long plane; // Ny * Mz
+    int Nx, Ny, Mz;
     double Nd;
     double2 *keep_phi, *keep_phi2, *keep_psi;
     long rkeep; // realization stride of keep_phi (ensembles)
     __device__ double2 operator()(int, long line, int n, double2 v) const
     {
         const bool in = line < plane; // padding lines read entry 0
-        const long i = (long)n * plane + (in ? line : 0);
-        const double F = filt[i];
+        const long lc = in ? line : 0;
+        const long i = (long)n * plane + lc;
+        // the mask at (n, j, k) of line j Mz + k: the smallest axis window
+        const int j = (int)(lc / Mz), k = (int)(lc - (long)j * Mz);
+        const double F = fmin(fmin(filt[n], filt[Nx + j]), filt[Nx + Ny + k]);
         const double2 w = make_double2(v.x * F, v.y * F);
         if (in && blockIdx.y == 0 && keep_phi) {
             keep_phi[blockIdx.z * rkeep + i] = w;
@@ -210,6 +215,7 @@ __global__ __launch_bounds__(RED_WG) void k_means(
 
 // update_phi_psi (src/toolbox.c:282-299); lap and p were parked in
 // result[N..2N) and result[3N..4N) by k_z_c2r_rho
+template <bool NT>
 __global__ __launch_bounds__(RED_WG) void k_update_phi_psi(
     const double *__restrict__ f, double *__restrict__ result, long N,
     long Ntot, double m2, const double *__restrict__ dscal, Ens e)
@@ -229,19 +235,19 @@ __global__ __launch_bounds__(RED_WG) void k_update_phi_psi(
     double2 *Rs = reinterpret_cast<double2 *>(result);
     const long H = N / 2;
     for (long i = (long)blockIdx.x * RED_WG + threadIdx.x; i < H; i += stride) {
-        const double2 ph = F[i], dph = F[H + i], ps = F[2 * H + i],
-                      dps = F[3 * H + i];
-        const double2 lap = Rs[H + i], pr = Rs[3 * H + i];
-        Rs[i] = dph;
-        Rs[H + i] = make_double2(
+        const double2 ph = ld2<NT>(F + i), dph = ld2<NT>(F + H + i),
+                      ps = ld2<NT>(F + 2 * H + i), dps = ld2<NT>(F + 3 * H + i);
+        const double2 lap = ld2<NT>(Rs + H + i), pr = ld2<NT>(Rs + 3 * H + i);
+        st2<NT>(Rs + i, dph);
+        st2<NT>(Rs + H + i, make_double2(
             (1.0 + 4.0 * ps.x) * lap.x / a2 - (h3 - 4.0 * dps.x) * dph.x -
                 (1.0 + 2.0 * ps.x) * (m2 * ph.x),
             (1.0 + 4.0 * ps.y) * lap.y / a2 - (h3 - 4.0 * dps.y) * dph.y -
-                (1.0 + 2.0 * ps.y) * (m2 * ph.y));
-        Rs[2 * H + i] = dps;
-        Rs[3 * H + i] = make_double2(
+                (1.0 + 2.0 * ps.y) * (m2 * ph.y)));
+        st2<NT>(Rs + 2 * H + i, dps);
+        st2<NT>(Rs + 3 * H + i, make_double2(
             0.5 * pr.x + (ps.x - 0.5) * pm - 4.0 * hubble * dps.x,
-            0.5 * pr.y + (ps.y - 0.5) * pm - 4.0 * hubble * dps.y);
+            0.5 * pr.y + (ps.y - 0.5) * pm - 4.0 * hubble * dps.y));
     }
 }
 
@@ -620,7 +626,11 @@ int rhs_core(ihg_ctx *c, double *f, double *result, int output, int kept,
         return hip_check(hipGetLastError(), "k_means");
     }));
     return timed(c, F_UPDATE, 10.0 * 8.0 * N * nr, [&]() {
-        hipLaunchKernelGGL(k_update_phi_psi,
+        // the streaming hint when the four blocks of all realizations
+        // outgrow the caches
+        const bool nt = 4.0 * 8.0 * (double)N * nr >= NT_BYTES;
+        hipLaunchKernelGGL((nt ? k_update_phi_psi<true>
+                               : k_update_phi_psi<false>),
                            dim3(nblocks(N, RED_WG * 4, 65536), 1, nr),
                            dim3(RED_WG), 0, c->stream, f, result, N, c->Ntot,
                            m2, c->dscal, e);
@@ -704,8 +714,8 @@ int rhs_filter(ihg_ctx *c, double *f, int nr)
         const LineSet ls = x_lineset(c);
         // one realization: keep phi_hat in W4 and Wc, psi_hat in Wc + M;
         // an ensemble (no output calls) keeps phi_hat in each W4
-        FilterFac fac{c->filter, (long)c->Ny * c->Mz, (double)N, W + 4 * M,
-                      nr > 1 ? nullptr : c->Wc,
+        FilterFac fac{c->filter, (long)c->Ny * c->Mz, c->Nx, c->Ny, c->Mz,
+                      (double)N, W + 4 * M, nr > 1 ? nullptr : c->Wc,
                       nr > 1 ? nullptr : c->Wc + M, e.w};
         // four fields in and out, three kept spectra out
         return timed(c, F_X_ROUNDTRIP, 11.0 * 16.0 * M * nr, [&]() {

@@ -13,6 +13,7 @@
 #include "dp853_tableau.h"
 #include "gsl_tableau.h"
 #include "ihg_internal.h"
+#include "mem.h"
 #include "reduce.h"
 
 namespace ihg {
@@ -41,6 +42,9 @@ static inline void rk4_extent(const ihg_ctx *c, long &nl, long &tail)
     }
 }
 
+// whether arrays of n doubles take the streaming hint
+static inline bool streaming(long n) { return 8.0 * (double)n >= NT_BYTES; }
+
 static inline unsigned stream_blocks(long npairs)
 {
     long b = (npairs + RED_WG - 1) / RED_WG;
@@ -48,7 +52,9 @@ static inline unsigned stream_blocks(long npairs)
     return (unsigned)(b > 262144 ? 262144 : b);
 }
 
-// out = y + dt * sum_j a_j k_j   (try_step's stage loops)
+// out = y + dt * sum_j a_j k_j   (try_step's stage loops); NT: streaming
+// loads and stores (arrays far larger than the caches)
+template <bool NT>
 __global__ __launch_bounds__(RED_WG) void k_dp_stage(
     double *__restrict__ out, const double *__restrict__ y, StageIn s,
     double dt, long nlive, long Ntot)
@@ -59,18 +65,19 @@ __global__ __launch_bounds__(RED_WG) void k_dp_stage(
     double2 *o2 = reinterpret_cast<double2 *>(out);
     for (long i = (long)blockIdx.x * RED_WG + threadIdx.x; i < npairs;
          i += stride) {
-        const double2 k0 = reinterpret_cast<const double2 *>(s.k[0])[i];
+        const double2 k0 = ld2<NT>(reinterpret_cast<const double2 *>(s.k[0]) + i);
         double ax = s.a[0] * k0.x, ay = s.a[0] * k0.y;
 #pragma unroll
         for (int j = 1; j < 12; ++j) {
             if (j < s.nk) {
-                const double2 kj = reinterpret_cast<const double2 *>(s.k[j])[i];
+                const double2 kj =
+                    ld2<NT>(reinterpret_cast<const double2 *>(s.k[j]) + i);
                 ax += s.a[j] * kj.x;
                 ay += s.a[j] * kj.y;
             }
         }
-        const double2 yy = y2[i];
-        o2[i] = make_double2(yy.x + dt * ax, yy.y + dt * ay);
+        const double2 yy = ld2<NT>(y2 + i);
+        st2<NT>(o2 + i, make_double2(yy.x + dt * ax, yy.y + dt * ay));
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         const long i = Ntot - 1;
@@ -83,6 +90,7 @@ __global__ __launch_bounds__(RED_WG) void k_dp_stage(
 
 // dense output: out = y + beta (y_new - y) + dt * sum_j w_j k_j, the
 // interpolant of an accepted step at one theta (weights from the host)
+template <bool NT>
 __global__ __launch_bounds__(RED_WG) void k_dp_dense(
     double *__restrict__ out, const double *__restrict__ y,
     const double *__restrict__ y_new, double beta, StageIn s, double dt,
@@ -99,14 +107,15 @@ __global__ __launch_bounds__(RED_WG) void k_dp_dense(
 #pragma unroll
         for (int j = 0; j < 12; ++j) {
             if (j < s.nk) {
-                const double2 kj = reinterpret_cast<const double2 *>(s.k[j])[i];
+                const double2 kj =
+                    ld2<NT>(reinterpret_cast<const double2 *>(s.k[j]) + i);
                 ax += s.a[j] * kj.x;
                 ay += s.a[j] * kj.y;
             }
         }
-        const double2 yy = y2[i], yn = n2[i];
-        o2[i] = make_double2(yy.x + beta * (yn.x - yy.x) + dt * ax,
-                             yy.y + beta * (yn.y - yy.y) + dt * ay);
+        const double2 yy = ld2<NT>(y2 + i), yn = ld2<NT>(n2 + i);
+        st2<NT>(o2 + i, make_double2(yy.x + beta * (yn.x - yy.x) + dt * ax,
+                                     yy.y + beta * (yn.y - yy.y) + dt * ay));
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         const long i = Ntot - 1;
@@ -119,6 +128,7 @@ __global__ __launch_bounds__(RED_WG) void k_dp_dense(
 
 // RK4 stage: out = y + dt * k / 2 (half) or y + dt * k, over the pairs of
 // [0, nlive) and the element `tail` (the scale factor; -1: none)
+template <bool NT>
 __global__ __launch_bounds__(RED_WG) void k_rk4_stage(
     double *__restrict__ out, const double *__restrict__ y,
     const double *__restrict__ k, double dt, int half, long nlive, long tail)
@@ -130,11 +140,12 @@ __global__ __launch_bounds__(RED_WG) void k_rk4_stage(
     double2 *o2 = reinterpret_cast<double2 *>(out);
     for (long i = (long)blockIdx.x * RED_WG + threadIdx.x; i < npairs;
          i += stride) {
-        const double2 yy = y2[i], kk = k2[i];
+        const double2 yy = ld2<NT>(y2 + i), kk = ld2<NT>(k2 + i);
         if (half)
-            o2[i] = make_double2(yy.x + dt * kk.x / 2.0, yy.y + dt * kk.y / 2.0);
+            st2<NT>(o2 + i, make_double2(yy.x + dt * kk.x / 2.0,
+                                         yy.y + dt * kk.y / 2.0));
         else
-            o2[i] = make_double2(yy.x + dt * kk.x, yy.y + dt * kk.y);
+            st2<NT>(o2 + i, make_double2(yy.x + dt * kk.x, yy.y + dt * kk.y));
     }
     if (tail >= 0 && blockIdx.x == 0 && threadIdx.x == 0) {
         const long i = tail;
@@ -143,6 +154,7 @@ __global__ __launch_bounds__(RED_WG) void k_rk4_stage(
 }
 
 // RK4 update: y += dt * (k1 + 2 k2 + 2 k3 + k4) / 6
+template <bool NT>
 __global__ __launch_bounds__(RED_WG) void k_rk4_final(
     double *__restrict__ y, const double *__restrict__ k1,
     const double *__restrict__ k2, const double *__restrict__ k3,
@@ -153,14 +165,14 @@ __global__ __launch_bounds__(RED_WG) void k_rk4_final(
     double2 *y2 = reinterpret_cast<double2 *>(y);
     for (long i = (long)blockIdx.x * RED_WG + threadIdx.x; i < npairs;
          i += stride) {
-        const double2 a = reinterpret_cast<const double2 *>(k1)[i];
-        const double2 b = reinterpret_cast<const double2 *>(k2)[i];
-        const double2 c = reinterpret_cast<const double2 *>(k3)[i];
-        const double2 d = reinterpret_cast<const double2 *>(k4)[i];
-        double2 yy = y2[i];
+        const double2 a = ld2<NT>(reinterpret_cast<const double2 *>(k1) + i);
+        const double2 b = ld2<NT>(reinterpret_cast<const double2 *>(k2) + i);
+        const double2 c = ld2<NT>(reinterpret_cast<const double2 *>(k3) + i);
+        const double2 d = ld2<NT>(reinterpret_cast<const double2 *>(k4) + i);
+        double2 yy = ld2<NT>(y2 + i);
         yy.x += dt * (a.x + 2.0 * b.x + 2.0 * c.x + d.x) / 6.0;
         yy.y += dt * (a.y + 2.0 * b.y + 2.0 * c.y + d.y) / 6.0;
-        y2[i] = yy;
+        st2<NT>(y2 + i, yy);
     }
     if (tail >= 0 && blockIdx.x == 0 && threadIdx.x == 0) {
         const long i = tail;
@@ -189,15 +201,16 @@ __device__ __forceinline__ void dp_finish_point(const FinishIn &f, long i,
     const double yerr2 = DP_ER[0] * k0 + DP_ER[5] * k5 + DP_ER[6] * k6 +
                          DP_ER[7] * k7 + DP_ER[8] * k8 + DP_ER[9] * k9 +
                          DP_ER[10] * k10 + DP_ER[11] * k11;
-    const double sk = atol + rtol * fmax(fabs(yv), fabs(ynew));
-    double t = yerr / sk;
+    const double inv = 1.0 / (atol + rtol * fmax(fabs(yv), fabs(ynew)));
+    double t = yerr * inv;
     e3 += t * t;
-    t = yerr2 / sk;
+    t = yerr2 * inv;
     e5 += t * t;
 }
 
 // the same for the elements 2i, 2i+1 (their error terms summed in that
 // order)
+template <bool NT>
 __device__ __forceinline__ void dp_finish_pair(const FinishIn &f, long i,
                                                double2 yv, double dt,
                                                double rtol, double atol,
@@ -208,8 +221,10 @@ __device__ __forceinline__ void dp_finish_pair(const FinishIn &f, long i,
 #pragma unroll
     for (int j = 0; j < 12; ++j)
         k[j] = reinterpret_cast<const double2 *>(f.k[j]);
-    const double2 k0 = k[0][i], k5 = k[5][i], k6 = k[6][i], k7 = k[7][i],
-                  k8 = k[8][i], k9 = k[9][i], k10 = k[10][i], k11 = k[11][i];
+    const double2 k0 = ld2<NT>(k[0] + i), k5 = ld2<NT>(k[5] + i),
+                  k6 = ld2<NT>(k[6] + i), k7 = ld2<NT>(k[7] + i),
+                  k8 = ld2<NT>(k[8] + i), k9 = ld2<NT>(k[9] + i),
+                  k10 = ld2<NT>(k[10] + i), k11 = ld2<NT>(k[11] + i);
     auto one = [&](double y0, double a0, double a5, double a6, double a7,
                    double a8, double a9, double a10, double a11) {
         const double bs = DP_B[0] * a0 + DP_B[5] * a5 + DP_B[6] * a6 +
@@ -220,10 +235,11 @@ __device__ __forceinline__ void dp_finish_pair(const FinishIn &f, long i,
         const double yerr2 = DP_ER[0] * a0 + DP_ER[5] * a5 + DP_ER[6] * a6 +
                              DP_ER[7] * a7 + DP_ER[8] * a8 + DP_ER[9] * a9 +
                              DP_ER[10] * a10 + DP_ER[11] * a11;
-        const double sk = atol + rtol * fmax(fabs(y0), fabs(yn));
-        double t = yerr / sk;
+        // one division per element: both ratios scale by 1 / sk
+        const double inv = 1.0 / (atol + rtol * fmax(fabs(y0), fabs(yn)));
+        double t = yerr * inv;
         e3 += t * t;
-        t = yerr2 / sk;
+        t = yerr2 * inv;
         e5 += t * t;
         return yn;
     };
@@ -232,6 +248,7 @@ __device__ __forceinline__ void dp_finish_pair(const FinishIn &f, long i,
 }
 
 // b-weighted update, embedded errors and the two sums of error()
+template <bool NT>
 __global__ __launch_bounds__(RED_WG) void k_dp_finish(
     const double *__restrict__ y, FinishIn f, double *__restrict__ ynew,
     double dt, double rtol, double atol, long nlive, long Ntot,
@@ -248,9 +265,9 @@ __global__ __launch_bounds__(RED_WG) void k_dp_finish(
     for (long i = (long)blockIdx.x * RED_WG + threadIdx.x; i < npairs;
          i += stride) {
         double2 yn;
-        const double2 yv = y2[i];
-        dp_finish_pair(f, i, yv, dt, rtol, atol, yn, e5, e3);
-        n2[i] = yn;
+        const double2 yv = ld2<NT>(y2 + i);
+        dp_finish_pair<NT>(f, i, yv, dt, rtol, atol, yn, e5, e3);
+        st2<NT>(n2 + i, yn);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         double yn;
@@ -279,6 +296,29 @@ __global__ __launch_bounds__(RED_WG) void k_sum2_final(
     if (threadIdx.x == 0) {
         out[0] = s0;
         out[1] = s1;
+    }
+}
+
+// sums of (a - b)^2 and (c - d)^2 over n elements (check_for_stiffness,
+// src/dopri853.c:550-560), block partials for k_sum2_final
+__global__ __launch_bounds__(RED_WG) void k_diff_norms(
+    const double *__restrict__ a, const double *__restrict__ b,
+    const double *__restrict__ c, const double *__restrict__ d, long n,
+    double *__restrict__ part)
+{
+    __shared__ double sh[8];
+    double s0 = 0.0, s1 = 0.0;
+    const long stride = (long)gridDim.x * RED_WG;
+    for (long i = (long)blockIdx.x * RED_WG + threadIdx.x; i < n; i += stride) {
+        const double u = a[i] - b[i], v = c[i] - d[i];
+        s0 += u * u;
+        s1 += v * v;
+    }
+    s0 = block_reduce<OpSum>(s0, sh, RED_WG);
+    s1 = block_reduce<OpSum>(s1, sh, RED_WG);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = s0;
+        part[2 * blockIdx.x + 1] = s1;
     }
 }
 
@@ -399,7 +439,9 @@ extern "C" int ihg_rk4_stage(ihg_ctx *c, double *out, const double *y,
     long nl, tail;
     rk4_extent(c, nl, tail);
     return timed(c, F_RK4, 3.0 * 8.0 * nl, [&]() {
-        hipLaunchKernelGGL(k_rk4_stage, dim3(stream_blocks(nl / 2)),
+        hipLaunchKernelGGL((streaming(nl) ? k_rk4_stage<true>
+                                          : k_rk4_stage<false>),
+                           dim3(stream_blocks(nl / 2)),
                            dim3(RED_WG), 0, c->stream, out, y, k, dt,
                            half ? 1 : 0, nl, tail);
         return hip_check(hipGetLastError(), "k_rk4_stage");
@@ -419,7 +461,9 @@ extern "C" int ihg_rk4_final(ihg_ctx *c, double *y, const double *k1,
     long nl, tail;
     rk4_extent(c, nl, tail);
     return timed(c, F_RK4, 6.0 * 8.0 * nl, [&]() {
-        hipLaunchKernelGGL(k_rk4_final, dim3(stream_blocks(nl / 2)),
+        hipLaunchKernelGGL((streaming(nl) ? k_rk4_final<true>
+                                          : k_rk4_final<false>),
+                           dim3(stream_blocks(nl / 2)),
                            dim3(RED_WG), 0, c->stream, y, k1, k2, k3, k4, dt,
                            nl, tail);
         return hip_check(hipGetLastError(), "k_rk4_final");
@@ -445,7 +489,9 @@ extern "C" int ihg_dp853_stage(ihg_ctx *c, int s, double dt, const double *y,
     in.nk = n;
     const long nl = live_len(c);
     return timed(c, F_DP_STAGE, (n + 2.0) * 8.0 * nl, [&]() {
-        hipLaunchKernelGGL(k_dp_stage, dim3(stream_blocks(nl / 2)),
+        hipLaunchKernelGGL((streaming(nl) ? k_dp_stage<true>
+                                          : k_dp_stage<false>),
+                           dim3(stream_blocks(nl / 2)),
                            dim3(RED_WG), 0, c->stream, out, y, in, dt, nl,
                            c->Ntot);
         return hip_check(hipGetLastError(), "k_dp_stage");
@@ -473,7 +519,9 @@ extern "C" int ihg_dp853_dense_stage(ihg_ctx *c, int s, double dt,
     in.nk = n;
     const long nl = live_len(c);
     return timed(c, F_DP_DENSE_STAGE, (n + 2.0) * 8.0 * nl, [&]() {
-        hipLaunchKernelGGL(k_dp_stage, dim3(stream_blocks(nl / 2)),
+        hipLaunchKernelGGL((streaming(nl) ? k_dp_stage<true>
+                                          : k_dp_stage<false>),
+                           dim3(stream_blocks(nl / 2)),
                            dim3(RED_WG), 0, c->stream, out, y, in, dt, nl,
                            c->Ntot);
         return hip_check(hipGetLastError(), "k_dp_stage");
@@ -522,7 +570,9 @@ extern "C" int ihg_dp853_dense(ihg_ctx *c, double theta, double dt,
     in.nk = n;
     const long nl = live_len(c);
     return timed(c, F_DP_DENSE, (n + 3.0) * 8.0 * nl, [&]() {
-        hipLaunchKernelGGL(k_dp_dense, dim3(stream_blocks(nl / 2)),
+        hipLaunchKernelGGL((streaming(nl) ? k_dp_dense<true>
+                                          : k_dp_dense<false>),
+                           dim3(stream_blocks(nl / 2)),
                            dim3(RED_WG), 0, c->stream, out, y, y_new, beta, in,
                            dt, nl, c->Ntot);
         return hip_check(hipGetLastError(), "k_dp_dense");
@@ -585,7 +635,9 @@ extern "C" int ihg_dp853_finish(ihg_ctx *c, double dt, const double *y,
     if (2 * nb + 2 > c->npartials) return set_error("partials buffer too small");
     // y and the 8 weighted stages in, y_new out
     IHG_TRY(timed(c, F_DP_FINISH, 10.0 * 8.0 * nl, [&]() {
-        hipLaunchKernelGGL(k_dp_finish, dim3((unsigned)nb), dim3(RED_WG), 0,
+        hipLaunchKernelGGL((streaming(nl) ? k_dp_finish<true>
+                                          : k_dp_finish<false>),
+                           dim3((unsigned)nb), dim3(RED_WG), 0,
                            c->stream, y, in, y_new, dt, rtol, atol, nl,
                            c->Ntot, c->partials);
         return hip_check(hipGetLastError(), "k_dp_finish");
@@ -653,7 +705,9 @@ extern "C" int ihg_dp853_finish_noalloc(void *stream, long nlive, long ntot,
     const long nb = finish_groups(nlive);
     // every partial k_sum2_final reads is written by k_dp_finish first, so
     // the scratch needs no clearing between calls
-    hipLaunchKernelGGL(k_dp_finish, dim3((unsigned)nb), dim3(RED_WG), 0, s,
+    hipLaunchKernelGGL((streaming(nlive) ? k_dp_finish<true>
+                                         : k_dp_finish<false>),
+                       dim3((unsigned)nb), dim3(RED_WG), 0, s,
                        y, in, y_new, dt, rtol, atol, nlive, ntot, part);
     IHG_LAUNCHED("k_dp_finish");
     hipLaunchKernelGGL(k_sum2_final, dim3(1), dim3(RED_WG), 0, s, part,
@@ -662,6 +716,29 @@ extern "C" int ihg_dp853_finish_noalloc(void *stream, long nlive, long ntot,
     IHG_HIP(hipMemcpyAsync(sums, part + NOALLOC_SUMS, 2 * sizeof(double),
                            hipMemcpyDeviceToHost, s));
     IHG_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+extern "C" int ihg_diff_norms(ihg_ctx *c, const double *a, const double *b,
+                              const double *cc, const double *d, size_t n,
+                              double out[2])
+{
+    if (!c || !a || !b || !cc || !d || !out) return set_error("NULL argument");
+    if (n < 1) return set_error("empty arrays");
+    long nb = ((long)n + RED_WG * 4 - 1) / (RED_WG * 4);
+    if (nb > 4096) nb = 4096;
+    if (2 * nb + 2 > c->npartials) return set_error("partials buffer too small");
+    hipLaunchKernelGGL(k_diff_norms, dim3((unsigned)nb), dim3(RED_WG), 0,
+                       c->stream, a, b, cc, d, (long)n, c->partials);
+    IHG_LAUNCHED("k_diff_norms");
+    hipLaunchKernelGGL(k_sum2_final, dim3(1), dim3(RED_WG), 0, c->stream,
+                       c->partials, (int)nb, c->dscal + 8);
+    IHG_LAUNCHED("k_sum2_final");
+    IHG_HIP(hipMemcpyAsync(c->hscal + 8, c->dscal + 8, 2 * sizeof(double),
+                           hipMemcpyDeviceToHost, c->stream));
+    IHG_HIP(hipStreamSynchronize(c->stream));
+    out[0] = c->hscal[8];
+    out[1] = c->hscal[9];
     return 0;
 }
 
@@ -728,7 +805,9 @@ extern "C" int ihg_gsl_stage(ihg_ctx *c, int method, int s, double dt,
     in.nk = n;
     const long nl = live_len(c);
     return timed(c, F_GSL_STAGE, (n + 2.0) * 8.0 * nl, [&]() {
-        hipLaunchKernelGGL(k_dp_stage, dim3(stream_blocks(nl / 2)),
+        hipLaunchKernelGGL((streaming(nl) ? k_dp_stage<true>
+                                          : k_dp_stage<false>),
+                           dim3(stream_blocks(nl / 2)),
                            dim3(RED_WG), 0, c->stream, out, y, in, dt, nl,
                            c->Ntot);
         return hip_check(hipGetLastError(), "k_dp_stage (gsl)");

@@ -41,7 +41,7 @@ H5_SYMBOLS = [
     "ih_h5_save", "ih_h5_status", "ih_h5_slices", "ih_h5_write_simple",
     "ih_h5_write_dp_stats", "ih_h5_write_followup", "ih_h5_close",
     "ih_h5_last_error", "ih_h5_read_followup", "ih_h5_read_timeslice",
-    "ih_h5_read", "ih_h5_read_string",
+    "ih_h5_read", "ih_h5_read_string", "ih_h5_buffered", "ih_h5_followup",
 ]
 
 _dptr = ctypes.POINTER(ctypes.c_double)
@@ -84,6 +84,8 @@ def h5_lib():
         "ih_h5_write_simple": (i, [_vp, ctypes.c_char_p, _dptr, sz]),
         "ih_h5_write_dp_stats": (i, [_vp, ctypes.POINTER(IhDpStats)]),
         "ih_h5_write_followup": (i, [_vp]),
+        "ih_h5_buffered": (ctypes.c_long, [_vp]),
+        "ih_h5_followup": (i, [_vp]),
         "ih_h5_close": (i, [_vp]),
         "ih_h5_last_error": (ctypes.c_char_p, []),
         "ih_h5_read_followup": (i, [_vp, ctypes.c_char_p]),
@@ -174,8 +176,18 @@ class H5Output:
         """stats: the dict Sim.run_dopri853 returns"""
         st = IhDpStats()
         for k, _ in IhDpStats._fields_:
-            setattr(st, k, stats[k])
+            if k in stats:
+                setattr(st, k, stats[k])
         _check(h5_lib().ih_h5_write_dp_stats(self.h, ctypes.byref(st)))
+
+    def wrapup_hooks(self):
+        """(buffered, followup, file) for ih_dp_opts: run_dopri853's wrap-up
+        saves the final slice only when this file's buffer holds slices, and
+        writes the followup state here when ENABLE_FOLLOWUP is set"""
+        from instaheat_amd import BUFFERED_FN, FOLLOWUP_FN
+        L = h5_lib()
+        return (ctypes.cast(L.ih_h5_buffered, BUFFERED_FN),
+                ctypes.cast(L.ih_h5_followup, FOLLOWUP_FN), self.h)
 
     def write_followup(self):
         sim = getattr(self, "_sim", None)

@@ -4,6 +4,9 @@
  */
 #include "instaheat_compat.h"
 
+#include <math.h>
+#include <string.h>
+
 struct evolution_flags evo_flags = {0, 0};
 
 static ih_sim *g_sim = NULL;
@@ -11,6 +14,9 @@ static ih_save_fn g_save = NULL;
 static void *g_user = NULL;
 static int g_status = 0;
 static long g_gsl_outputs = 1000; /* GSL_OUTPUT_NUMBER, src/parameters.sh:33 */
+static long (*g_buffered)(void *) = NULL;
+static int (*g_followup)(void *) = NULL;
+static void *g_file = NULL;
 
 void ih_compat_bind(ih_sim *s) { g_sim = s; }
 
@@ -18,6 +24,14 @@ void ih_compat_save_hook(ih_save_fn save, void *user)
 {
     g_save = save;
     g_user = user;
+}
+
+void ih_compat_file_hooks(long (*buffered)(void *), int (*followup)(void *),
+                          void *file)
+{
+    g_buffered = buffered;
+    g_followup = followup;
+    g_file = file;
 }
 
 int ih_compat_status(void) { return g_status; }
@@ -67,7 +81,12 @@ void run_rk4(void)
 void run_dopri853(void)
 {
     if (!bound()) return;
-    long n = ih_run_dopri853(g_sim, NULL, g_save, g_user, NULL);
+    ih_dp_opts opts;
+    memset(&opts, 0, sizeof opts);
+    opts.buffered = g_buffered;
+    opts.followup = g_followup;
+    opts.file = g_file;
+    long n = ih_run_dopri853(g_sim, &opts, g_save, g_user, NULL);
     g_status = n < 0 ? (int)n : 0;
 }
 
@@ -79,4 +98,29 @@ void run_gsl_stepper(void)
     if (!bound()) return;
     long n = ih_run_gsl(g_sim, g_gsl_outputs, g_save, g_user, NULL);
     g_status = n < 0 ? (int)n : 0;
+}
+
+/* prepare_and_save_timeslice, src/toolbox.c:96-104 */
+void prepare_and_save_timeslice(void)
+{
+    if (!bound()) return;
+    g_status = ih_prepare_and_save_timeslice(g_sim, g_save, g_user);
+}
+
+/* mean / variance, src/toolbox.c:726-754, of device arrays; NaN (and the
+ * error code in ih_compat_status) on failure */
+double mean(const double *f, const size_t N)
+{
+    double m = NAN;
+    if (!bound()) return m;
+    g_status = ihg_mean(ih_gpu(g_sim), f, N, &m);
+    return g_status ? NAN : m;
+}
+
+double variance(const double mean, const double *f, const size_t N)
+{
+    double v = NAN;
+    if (!bound()) return v;
+    g_status = ihg_variance(ih_gpu(g_sim), mean, f, N, &v);
+    return g_status ? NAN : v;
 }

@@ -571,6 +571,14 @@ int ih_h5_write_followup(ih_h5 *h)
     return rc;
 }
 
+long ih_h5_buffered(void *h5)
+{
+    const ih_h5 *h = h5;
+    return h ? h->index : 0;
+}
+
+int ih_h5_followup(void *h5) { return ih_h5_write_followup(h5); }
+
 /* h5_close, src/io.c:472-488, after main's last writes (src/main.c:99-106):
  * runtime_total and calls_rhs, unless written already */
 int ih_h5_close(ih_h5 *h)
@@ -786,7 +794,11 @@ int ih_h5_read_followup(ih_sim *s, const char *path)
         if (!rc) rc = ihg_sync(ih_gpu(s));
         if (rc) rc = fail(rc, "upload: %s", ihg_last_error());
     }
-    if (!rc) ih_set_time(s, times[nt - 1]);
+    if (!rc) {
+        /* pars.t.ti and pars.t.t (src/io.c:783-784) */
+        ih_set_time(s, times[nt - 1]);
+        ih_set_ti(s, times[nt - 1]);
+    }
     free(times);
     free(y);
     return rc;
@@ -844,7 +856,9 @@ int ih_h5_read_timeslice(ih_sim *s, const char *path, double t, double *t_found)
         if (rc) rc = fail(rc, "upload: %s", ihg_last_error());
     }
     if (!rc) {
+        /* pars.t.ti and pars.t.t (src/io.c:674-675) */
         ih_set_time(s, times[index]);
+        ih_set_ti(s, times[index]);
         if (t_found) *t_found = times[index];
     }
     free(times);

@@ -37,6 +37,7 @@ struct ih_sim {
     ih_dp_record hist[IH_HISTORY];
     long n_hist;
     const double *save_y; /* the state of the save() call in progress */
+    double t_created;     /* wall time of ih_create (MAX_RUNTIME's origin) */
 };
 
 static double wall_time(void)
@@ -71,6 +72,9 @@ void ih_config_defaults(ih_config *cfg)
     cfg->max_dt_hubble_fraction = 1.0e-2;
     cfg->skip = 1;
     cfg->a_initial = 1.0;
+    cfg->max_runtime = 172500.0;
+    cfg->enable_followup = 0;
+    cfg->enable_stiffness_check = 0;
 }
 
 void ih_destroy(ih_sim *s)
@@ -133,6 +137,7 @@ int ih_create(const ih_config *cfg, ih_sim **out)
         return rc;
     }
     s->t = cfg->ti;
+    s->t_created = wall_time();
     ihg_set_field(s->g, s->field);
     *out = s;
     return 0;
@@ -149,6 +154,7 @@ void ih_set_time(ih_sim *s, double t)
     s->t = t;
     s->n_hist = 0;
 }
+void ih_set_ti(ih_sim *s, double ti) { s->cfg.ti = ti; }
 void ih_set_dt(ih_sim *s, double dt) { s->cfg.dt = dt; }
 void ih_set_max_steps(ih_sim *s, long max_steps) { s->cfg.max_steps = max_steps; }
 void ih_set_tf(ih_sim *s, double tf) { s->cfg.tf = tf; }
@@ -250,6 +256,12 @@ static int prepare_and_save(ih_sim *s, ih_save_fn save, void *user)
     s->evo_flags.output = 0;
     if (rc) return rc;
     return do_save(s, save, user, s->t);
+}
+
+int ih_prepare_and_save_timeslice(ih_sim *s, ih_save_fn save, void *user)
+{
+    if (!s) return IHG_EINVAL;
+    return prepare_and_save(s, save, user);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -409,6 +421,22 @@ int ih_success(ih_ctrl *ctl, const ih_config *cfg, double err, double *dt,
     return 0;
 }
 
+/* check_for_stiffness, src/dopri853.c:550-580 (dtlamb keeps its last value
+ * when den is 0) */
+int ih_stiffness_update(ih_stiffness *st, double dt, double num, double den)
+{
+    if (den > 0.0) st->dtlamb = dt * sqrt(num / den);
+    if (st->dtlamb > 6.1) {
+        st->nonstiff = 0;
+        st->stiff += 1;
+        if (st->stiff == 15) return 1;
+    } else {
+        st->nonstiff += 1;
+        if (st->nonstiff == 6) st->stiff = 0;
+    }
+    return 0;
+}
+
 double ih_error_norm(double e5, double e3, double dt, size_t ntot)
 {
     double deno = e5 + 0.01 * e3;
@@ -447,8 +475,9 @@ long ih_run_dopri853(ih_sim *s, const ih_dp_opts *opts, ih_save_fn save,
     ih_ctrl ctl = {1.0e-4, c->dt, 0};
     double dt = c->dt, last_save_t = s->t;
     long n_stp, n_ok = 0, n_bad = 0, attempts = 0, next_out = 0;
-    int rc;
+    int rc, stop = 0;
     const int filter = c->grid.enable_filter ? 1 : 0;
+    ih_stiffness stiff = {0.0, 0, 0};
 
     rc = prepare_and_save(s, save, user);
     if (rc) return rc;
@@ -465,7 +494,10 @@ long ih_run_dopri853(ih_sim *s, const ih_dp_opts *opts, ih_save_fn save,
     const double t_start = wall_time();
     for (n_stp = 0;; ++n_stp) {
         if (opts->forced_dt) {
-            if (n_stp >= opts->n_forced) break;
+            if (n_stp >= opts->n_forced) {
+                stop = IH_STOP_FORCED;
+                break;
+            }
             dt = opts->forced_dt[n_stp];
         } else if (s->t + dt * 1.0001 > c->tf) {
             dt = c->tf - s->t;
@@ -523,7 +555,10 @@ long ih_run_dopri853(ih_sim *s, const ih_dp_opts *opts, ih_save_fn save,
                 break;
             }
         }
-        if (underflow) break;
+        if (underflow) {
+            stop = IH_STOP_UNDERFLOW;
+            break;
+        }
         s->evo_flags.filter = filter;
         if ((n_stp + 1) % c->skip == 0) s->evo_flags.output = 1;
         const int output = s->evo_flags.output;
@@ -543,6 +578,29 @@ long ih_run_dopri853(ih_sim *s, const ih_dp_opts *opts, ih_save_fn save,
         s->evo_flags.output = 0;
         s->evo_flags.filter = 0;
         if (rc) return rc;
+        /* check_for_stiffness (src/dopri853.c:295-299): the FSAL derivative
+         * against stage 12's, the filtered y_new against stage 12's input,
+         * with the step's first trial size; n_ok before this step's count */
+        if (c->enable_stiffness_check && ((n_ok % 10) == 0 || stiff.stiff > 0)) {
+            double nd[2];
+            rc = ihg_diff_norms(s->g, s->dfield_new, s->stage[11],
+                                s->field_new, s->k_tmp, s->Ntot, nd);
+            if (rc) return rc;
+            if (ih_stiffness_update(&stiff, dt, nd[0], nd[1])) {
+                if (stats) {
+                    memset(stats, 0, sizeof *stats);
+                    stats->n_stp = n_stp;
+                    stats->n_ok = n_ok;
+                    stats->n_bad = n_bad;
+                    stats->attempts = attempts;
+                    stats->t = s->t;
+                    stats->stop = IH_STOP_STIFF;
+                    stats->dtlamb = stiff.dtlamb;
+                    stats->calls_rhs = ihg_calls_rhs(s->g);
+                }
+                return IH_ESTIFF;
+            }
+        }
         /* field = field_new, dfield = dfield_new (as a swap) */
         double *tmp = s->field;
         s->field = s->field_new;
@@ -578,17 +636,38 @@ long ih_run_dopri853(ih_sim *s, const ih_dp_opts *opts, ih_save_fn save,
             if (rc) return rc;
             last_save_t = s->t;
         }
-        if (s->t >= c->tf) break;
-        if (n_stp >= c->max_steps) break;
-        if (!opts->forced_dt && fabs(ctl.dt_next) <= c->dt_min) break;
+        if (s->t >= c->tf) {
+            stop = IH_STOP_FINAL_TIME;
+            break;
+        }
+        if (n_stp >= c->max_steps) {
+            stop = IH_STOP_MAX_STEPS;
+            break;
+        }
+        if (c->max_runtime > 0.0 &&
+            wall_time() - s->t_created > c->max_runtime) {
+            stop = IH_STOP_MAX_RUNTIME;
+            break;
+        }
+        if (!opts->forced_dt && fabs(ctl.dt_next) <= c->dt_min) {
+            stop = IH_STOP_DT_MIN;
+            break;
+        }
         if (!opts->forced_dt) dt = ctl.dt_next;
     }
     rc = ihg_sync(s->g);
     if (rc) return rc;
     const double t_end = wall_time();
-    /* wrap_up_dopri853, src/dopri853.c:223-228 */
-    if (last_save_t < s->t) {
+    /* wrap_up_dopri853, src/dopri853.c:223-231: the final slice when the
+     * file buffer holds slices and the last one is older than t (without a
+     * file: whenever the last save is older than t), then the followup */
+    const int pending = opts->buffered ? opts->buffered(opts->file) != 0 : 1;
+    if (pending && last_save_t < s->t) {
         rc = prepare_and_save(s, save, user);
+        if (rc) return rc;
+    }
+    if (c->enable_followup && opts->followup) {
+        rc = opts->followup(opts->file);
         if (rc) return rc;
     }
     if (stats) {
@@ -601,6 +680,8 @@ long ih_run_dopri853(ih_sim *s, const ih_dp_opts *opts, ih_save_fn save,
         stats->err_old = ctl.err_old;
         stats->integration_seconds = t_end - t_start;
         stats->calls_rhs = ihg_calls_rhs(s->g);
+        stats->stop = stop;
+        stats->dtlamb = stiff.dtlamb;
     }
     return attempts;
 }

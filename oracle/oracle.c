@@ -1069,6 +1069,21 @@ static double dp_error(const orc_ctx *c, const dp_bufs *b, const double *field,
     return dt * err * sqrt(1.0 / (c->Ntot * deno));
 }
 
+/* check_for_stiffness, src/dopri853.c:550-580 */
+int orc_stiffness_update(orc_stiffness *st, double dt, double num, double den)
+{
+    if (den > 0.0) st->dtlamb = dt * sqrt(num / den);
+    if (st->dtlamb > 6.1) {
+        st->nonstiff = 0;
+        st->stiff += 1;
+        if (st->stiff == 15) return 1;
+    } else {
+        st->nonstiff += 1;
+        if (st->nonstiff == 6) st->stiff = 0;
+    }
+    return 0;
+}
+
 long orc_run_dopri853(orc_ctx *c, double *field, double *dfield,
                       const orc_dp_opts *o, orc_dp_stats *st,
                       orc_save_fn save, void *user, double *dt_log,
@@ -1087,7 +1102,8 @@ long orc_run_dopri853(orc_ctx *c, double *field, double *dfield,
     orc_ctrl ctl = {1.0e-4, o->dt, 0};
     double t = o->ti, dt = o->dt, dt_did = 0.0, last_save_t = o->ti;
     long n_stp, n_ok = 0, n_bad = 0, attempts = 0;
-    int filter = c->p.enable_filter ? 1 : 0;
+    int filter = c->p.enable_filter ? 1 : 0, stiff_stop = 0;
+    orc_stiffness stiff = {0.0, 0, 0};
     c->field = field;
 
     /* prepare_and_save_timeslice, src/toolbox.c:96-104 */
@@ -1126,6 +1142,20 @@ long orc_run_dopri853(orc_ctx *c, double *field, double *dfield,
         if (underflow) break;
         int output = ((n_stp + 1) % o->skip == 0) ? 1 : 0;
         orc_mk_rhs(c, t + h, b.field_new, b.dfield_new, filter, output);
+        /* src/dopri853.c:295-299 */
+        if (o->enable_stiffness_check && ((n_ok % 10) == 0 || stiff.stiff > 0)) {
+            double num = 0.0, den = 0.0;
+            for (size_t i = 0; i < Ntot; ++i) {
+                double tmp = b.dfield_new[i] - b.k[11][i];
+                num += tmp * tmp;
+                tmp = b.field_new[i] - b.k_tmp[i];
+                den += tmp * tmp;
+            }
+            if (orc_stiffness_update(&stiff, dt, num, den)) {
+                stiff_stop = 1; /* the reference exits here */
+                break;
+            }
+        }
         memcpy(field, b.field_new, Ntot * sizeof(double));
         memcpy(dfield, b.dfield_new, Ntot * sizeof(double));
         t += (dt_did = h);
@@ -1142,8 +1172,9 @@ long orc_run_dopri853(orc_ctx *c, double *field, double *dfield,
         if (!o->forced_dt && fabs(ctl.dt_next) <= o->dt_min) break;
         if (!o->forced_dt) dt = ctl.dt_next;
     }
-    /* wrap_up_dopri853, src/dopri853.c:223-228 */
-    if (last_save_t < t) {
+    /* wrap_up_dopri853, src/dopri853.c:223-228 (no file buffer here: the
+     * final slice whenever the last save is older than t) */
+    if (!stiff_stop && last_save_t < t) {
         orc_mk_rhs(c, t, field, dfield, 1, 1);
         do_save(c, save, user, t, field, obs);
     }
@@ -1152,6 +1183,8 @@ long orc_run_dopri853(orc_ctx *c, double *field, double *dfield,
         st->n_ok = n_ok;
         st->n_bad = n_bad;
         st->t = t;
+        st->dtlamb = stiff.dtlamb;
+        st->stiff = stiff_stop;
     }
     for (int s = 1; s < 12; ++s) free(b.k[s]);
     free(b.k_tmp);

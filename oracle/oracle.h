@@ -84,13 +84,24 @@ typedef struct orc_dp_opts {
     long skip;                   /* TIME_STEP_SKIPS */
     const double *forced_dt;     /* if non-NULL: accept exactly these steps */
     long n_forced;
+    int enable_stiffness_check;  /* ENABLE_STIFFNESSCHECK */
 } orc_dp_opts;
 
 typedef struct orc_dp_stats {
     long n_stp, n_ok, n_bad;
     double t;
     double last_err;
+    double dtlamb;               /* dp.dtlamb of the stiffness check */
+    int stiff;                   /* 1: stopped by the stiffness check */
 } orc_dp_stats;
+
+/* check_for_stiffness's counters (src/dopri853.c:550-580); returns 1 on
+ * the 15th stiff verdict */
+typedef struct orc_stiffness {
+    double dtlamb;
+    int stiff, nonstiff;
+} orc_stiffness;
+int orc_stiffness_update(orc_stiffness *st, double dt, double num, double den);
 
 /* run_dopri853 (src/dopri853.c:112-158).  dt_log/err_log (may be NULL)
  * receive every attempted dt and its error norm, up to log_cap entries;

@@ -39,13 +39,20 @@ class DpOpts(ctypes.Structure):
         ("max_dt_hubble_fraction", ctypes.c_double),
         ("skip", ctypes.c_long),
         ("forced_dt", _dptr), ("n_forced", ctypes.c_long),
+        ("enable_stiffness_check", ctypes.c_int),
     ]
 
 
 class DpStats(ctypes.Structure):
     _fields_ = [("n_stp", ctypes.c_long), ("n_ok", ctypes.c_long),
                 ("n_bad", ctypes.c_long), ("t", ctypes.c_double),
-                ("last_err", ctypes.c_double)]
+                ("last_err", ctypes.c_double), ("dtlamb", ctypes.c_double),
+                ("stiff", ctypes.c_int)]
+
+
+class Stiffness(ctypes.Structure):
+    _fields_ = [("dtlamb", ctypes.c_double), ("stiff", ctypes.c_int),
+                ("nonstiff", ctypes.c_int)]
 
 
 class Ctrl(ctypes.Structure):
@@ -118,6 +125,10 @@ def _bind(L):
                               ctypes.c_double, _dptr, ctypes.c_double]
     L.orc_success.restype = ctypes.c_int
     L.orc_dp_tableau.argtypes = [_dptr] * 5
+    L.orc_stiffness_update.argtypes = [ctypes.POINTER(Stiffness),
+                                       ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_double]
+    L.orc_stiffness_update.restype = ctypes.c_int
     return L
 
 
@@ -252,7 +263,8 @@ class Oracle:
     def run_dopri853(self, field, dfield, ti, tf, dt, rtol=1e-10, atol=1e-14,
                      dt_min=1e-9, max_steps=10**15, beta=0.0, safe=0.9,
                      minscale=0.333, maxscale=3.0, hubble_frac=1e-2, skip=1,
-                     forced=None, saves=None, log_cap=100000):
+                     forced=None, saves=None, log_cap=100000,
+                     stiffness_check=False):
         self.set_field(field)
         fz = None
         if forced is not None:
@@ -260,7 +272,7 @@ class Oracle:
         o = DpOpts(ti, tf, dt, dt_min, min(max_steps, 2**62), beta, safe,
                    minscale, maxscale, rtol, atol, hubble_frac, skip,
                    _p(fz) if fz is not None else None,
-                   len(fz) if fz is not None else 0)
+                   len(fz) if fz is not None else 0, int(bool(stiffness_check)))
         st = DpStats()
         dts = np.zeros(log_cap)
         errs = np.zeros(log_cap)
@@ -278,7 +290,8 @@ class Oracle:
         return {"attempts": n, "dt": dts[:m], "err": errs[:m],
                 "accepted": acc[:m].astype(bool), "n_stp": st.n_stp,
                 "n_ok": st.n_ok, "n_bad": st.n_bad, "t": st.t,
-                "last_err": st.last_err}
+                "last_err": st.last_err, "dtlamb": st.dtlamb,
+                "stiff": bool(st.stiff)}
 
 
 def tableau():
