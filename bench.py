@@ -279,38 +279,54 @@ def run_gpu(args, wl):
 
 
 def ensemble_bench(steps):
-    """The batched form: realization-steps per second of the mixed ensemble
-    with one graph launch per step and size against eager launches, with the
-    ensemble mean and variance of the GW spectrum per size (graph run)."""
+    """The batched form: realization-steps per second of the mixed ensemble,
+    per size on its own stream (eager launches, one graph per size and step,
+    graphs with the autotuned strided passes) and grouped (every pass of all
+    sizes in one launch, eager and as one graph per step), with the ensemble
+    mean and variance of the GW spectrum per size (grouped graph run)."""
     from instaheat_amd import ensemble
     out = {}
-    for name, graph, tune in (("eager", False, False), ("graph", True, False),
-                              ("tuned", True, True)):
+    for name, graph, tune, grouped in (("eager", False, False, False),
+                                       ("graph", True, False, False),
+                                       ("tuned", True, True, False),
+                                       ("grouped", False, False, True),
+                                       ("grouped_graph", True, False, True)):
         out[name] = ensemble.run_mixed(nsteps=steps, dt=0.005, use_graph=graph,
-                                       autotune=tune)
-    g, e, t = out["graph"], out["eager"], out["tuned"]
-    return {"metric": "realization-steps/s of the mixed ensemble "
-                      "(48 x 32^3 + 16 x 64^3 + 4 x 128^3, FP64 RK4 dt 0.005, GW on)",
-            "steps": steps,
-            "graph": {"value": round(g["realization_steps_per_s"], 2),
-                      "wall_s": round(g["wall_s"], 3), "rhs_evals": g["rhs_evals"]},
-            "eager": {"value": round(e["realization_steps_per_s"], 2),
-                      "wall_s": round(e["wall_s"], 3)},
-            "graph_over_eager": round(g["realization_steps_per_s"] /
-                                      e["realization_steps_per_s"], 3),
-            "tuned": {"value": round(t["realization_steps_per_s"], 2),
-                      "wall_s": round(t["wall_s"], 3),
-                      "candidates_yx": t["autotuned"]},
-            "tuned_over_eager": round(t["realization_steps_per_s"] /
-                                      e["realization_steps_per_s"], 3),
-            # the GW spectra agree to the rounding of their atomic bin sums
-            "graph_matches_eager": all(np.allclose(g["sizes"][n]["gw_mean"],
-                                                   e["sizes"][n]["gw_mean"],
-                                                   rtol=1e-12, atol=0)
-                                       for n in g["sizes"]),
-            "gw_spectrum": {str(n): {"nreal": v["nreal"], "t": v["t"],
-                                     "mean": v["gw_mean"], "var": v["gw_var"]}
-                            for n, v in g["sizes"].items()}}
+                                       autotune=tune, grouped=grouped,
+                                       keep_states=name in ("eager", "grouped_graph"))
+        print(f"ensemble {name}: {out[name]['realization_steps_per_s']:.1f} "
+              f"realization-steps/s", file=sys.stderr, flush=True)
+    e = out["eager"]
+
+    def rate(name):
+        r = out[name]
+        return {"value": round(r["realization_steps_per_s"], 2),
+                "wall_s": round(r["wall_s"], 3),
+                "ms_per_step": round(1e3 * r["wall_s"] / steps, 3),
+                "over_eager": round(r["realization_steps_per_s"] /
+                                    e["realization_steps_per_s"], 3)}
+    gg = out["grouped_graph"]
+    res = {"metric": "realization-steps/s of the mixed ensemble "
+                     "(48 x 32^3 + 16 x 64^3 + 4 x 128^3, FP64 RK4 dt 0.005, GW on)",
+           "steps": steps,
+           "value": round(gg["realization_steps_per_s"], 2),
+           "form": "grouped_graph (one launch per pass for all sizes, one graph per step)",
+           "rhs_evals": gg["rhs_evals"]}
+    for name in out:
+        res[name] = rate(name)
+    res["tuned"]["candidates_yx"] = out["tuned"]["autotuned"]
+    # the grouped steps equal the per-size steps bit for bit; the GW spectra
+    # agree to the rounding of their atomic bin sums
+    res["grouped_states_equal_per_size"] = all(
+        np.array_equal(gg["sizes"][n]["states"], e["sizes"][n]["states"])
+        for n in e["sizes"])
+    res["grouped_spectra_match_per_size"] = all(
+        np.allclose(gg["sizes"][n]["gw_mean"], e["sizes"][n]["gw_mean"],
+                    rtol=1e-12, atol=0) for n in e["sizes"])
+    res["gw_spectrum"] = {str(n): {"nreal": v["nreal"], "t": v["t"],
+                                   "mean": v["gw_mean"], "var": v["gw_var"]}
+                          for n, v in gg["sizes"].items()}
+    return res
 
 
 def measured_traffic(kernel, grid):

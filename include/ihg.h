@@ -227,6 +227,35 @@ int ihg_ensemble(const ihg_ctx *c, long out[2]);
 int ihg_mk_rhs_ensemble(ihg_ctx *c, double t, double *f, double *result,
                         unsigned flags);
 
+/* ---- groups: ensembles of several grid sizes in one launch per pass -----
+ * (no counterpart in the reference).  A group holds 1 to 4 ensemble (or
+ * single) contexts of cubic grids of 16, 32, 64 or 128 points, FP64.
+ * ihg_group_mk_rhs does what ihg_mk_rhs_ensemble does on every context, and
+ * ihg_group_rk4_stage / ihg_group_rk4_final what ihg_rk4_stage /
+ * ihg_rk4_final do, but each pass of all contexts is ONE launch: its work-
+ * groups come from a table of segments, one per context, each running the
+ * per-context kernel's body on its own grid, so the values are those of the
+ * per-context calls bit for bit.  Array arguments hold one pointer per
+ * context, in the order of ihg_group_create.  Launches go to the FIRST
+ * context's stream (capture them with ihg_graph_begin on that context); the
+ * other contexts' streams must be idle (ihg_sync) before grouped calls and
+ * are not ordered after them.  The strided passes use the default line
+ * plans (autotuned choices of the contexts are not applied), and the live
+ * profiler does not time grouped launches. */
+typedef struct ihg_group ihg_group;
+int ihg_group_create(ihg_ctx *const *ctxs, int n, ihg_group **out);
+int ihg_group_destroy(ihg_group *g);
+/* grouped launches made so far */
+long ihg_group_launches(const ihg_group *g);
+/* flags: IHG_FILTER only (applied to the contexts with the filter on) */
+int ihg_group_mk_rhs(ihg_group *g, double t, double *const *f,
+                     double *const *result, unsigned flags);
+int ihg_group_rk4_stage(ihg_group *g, double *const *out, double *const *y,
+                        double *const *k, double dt, int half);
+int ihg_group_rk4_final(ihg_group *g, double *const *y, double *const *k1,
+                        double *const *k2, double *const *k3,
+                        double *const *k4, double dt);
+
 /* HIP graphs of a context's stream: capture a sequence of library calls
  * that neither synchronises nor allocates (with the live profiler off),
  * replay it as one launch; ihg_graph_join puts independent graphs (e.g. one

@@ -286,6 +286,19 @@ int ih_ens_init(ih_ens *e, const uint64_t *seeds, const double *amplitudes,
 int ih_ens_capture_step(ih_ens *e, double dt, ihg_graph **g);
 int ih_ens_run_rk4(ih_ens *const *es, int n, long nsteps, double dt,
                    int use_graph);
+/* the same steps with every pass of all ensembles in one launch (an
+ * ihg_group over their contexts, on the first ensemble's stream); with
+ * use_graph one RK4 step of all of them is captured once as a single HIP
+ * graph and replayed once per step.  The end states equal ih_ens_run_rk4's
+ * bit for bit when the contexts keep their default strided passes (no
+ * autotuned choice). */
+int ih_ens_run_rk4_grouped(ih_ens *const *es, int n, long nsteps, double dt,
+                           int use_graph);
+/* one RK4 step of all the ensembles, grouped, captured as one HIP graph to
+ * replay on the first ensemble's context (ihg_graph_launch); the replays do
+ * not advance the ensembles' times */
+int ih_ens_capture_step_grouped(ih_ens *const *es, int n, double dt,
+                                ihg_graph **g);
 /* the observables of realization r (an output mk_rhs on a copy of it) */
 int ih_ens_observables(ih_ens *e, int r, double *obs, size_t len);
 

@@ -169,6 +169,8 @@ IHG_SYMBOLS = [
     "ihg_set_fft_candidates", "ihg_create_ensemble", "ihg_ensemble",
     "ihg_mk_rhs_ensemble", "ihg_graph_begin", "ihg_graph_end",
     "ihg_graph_join", "ihg_graph_launch", "ihg_graph_destroy",
+    "ihg_group_create", "ihg_group_destroy", "ihg_group_launches",
+    "ihg_group_mk_rhs", "ihg_group_rk4_stage", "ihg_group_rk4_final",
     "ihg_mlp_nparams", "ihg_mlp_create", "ihg_mlp_destroy",
     "ihg_mlp_get_params", "ihg_mlp_set_params", "ihg_mlp_forward",
     "ihg_mlp_grad", "ihg_mlp_train",
@@ -184,7 +186,9 @@ HOST_SYMBOLS = [
     "ih_mlp_propose", "ih_ens_create", "ih_ens_destroy", "ih_ens_gpu",
     "ih_ens_state", "ih_ens_stride", "ih_ens_nreal", "ih_ens_time",
     "ih_ens_calls_rhs", "ih_ens_init", "ih_ens_capture_step",
-    "ih_ens_run_rk4", "ih_ens_observables", "ih_run_gsl", "ih_gsl_hadjust",
+    "ih_ens_run_rk4", "ih_ens_run_rk4_grouped", "ih_ens_capture_step_grouped",
+    "ih_ens_observables",
+    "ih_run_gsl", "ih_gsl_hadjust",
 ]
 
 _lib = None
@@ -264,6 +268,14 @@ def lib():
         "ihg_graph_end": (i, [_vp, ctypes.POINTER(_vp)]),
         "ihg_graph_join": (i, [ctypes.POINTER(_vp), i, ctypes.POINTER(_vp)]),
         "ihg_graph_launch": (i, [_vp, _vp]),
+        "ihg_group_create": (i, [ctypes.POINTER(_vp), i, ctypes.POINTER(_vp)]),
+        "ihg_group_destroy": (i, [_vp]),
+        "ihg_group_launches": (ctypes.c_long, [_vp]),
+        "ihg_group_mk_rhs": (i, [_vp, d, ctypes.POINTER(_vp),
+                                 ctypes.POINTER(_vp), ctypes.c_uint]),
+        "ihg_group_rk4_stage": (i, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                    ctypes.POINTER(_vp), d, i]),
+        "ihg_group_rk4_final": (i, [_vp] + [ctypes.POINTER(_vp)] * 5 + [d]),
         "ihg_graph_destroy": (i, [_vp]),
         "ihg_mlp_nparams": (i, []),
         "ihg_mlp_create": (i, [ctypes.POINTER(_vp), _fptr, ctypes.c_ulong]),
@@ -335,6 +347,10 @@ def host_lib():
         "ih_ens_init": (i, [_vp, ctypes.POINTER(ctypes.c_uint64), _dptr, d, d, d]),
         "ih_ens_capture_step": (i, [_vp, d, ctypes.POINTER(_vp)]),
         "ih_ens_run_rk4": (i, [ctypes.POINTER(_vp), i, ctypes.c_long, d, i]),
+        "ih_ens_run_rk4_grouped": (i, [ctypes.POINTER(_vp), i, ctypes.c_long, d,
+                                       i]),
+        "ih_ens_capture_step_grouped": (i, [ctypes.POINTER(_vp), i, d,
+                                            ctypes.POINTER(_vp)]),
         "ih_ens_observables": (i, [_vp, i, _dptr, ctypes.c_size_t]),
         "ih_proposer_features": (None, [ctypes.POINTER(IhDpRecord),
                                         ctypes.c_long, _dptr]),

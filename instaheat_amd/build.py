@@ -35,7 +35,8 @@ C_FLAGS = ["-std=c99", "-O2", "-fPIC", "-Wall", "-Wextra",
            "-I", os.path.join(ROOT, "include")]
 
 HIP_SOURCES = ["ihg_api.hip", "ihg_fft.hip", "ihg_rhs.hip", "ihg_obs.hip",
-               "ihg_step.hip", "ihg_mlp.hip", "ihg_graph.hip", "ihg_ic.hip"]
+               "ihg_step.hip", "ihg_mlp.hip", "ihg_graph.hip", "ihg_ic.hip",
+               "ihg_group.hip"]
 HOST_SOURCES = ["instaheat.c", "compat.c"]
 HDF5_ROOT = os.environ.get("IH_HDF5_ROOT", "/opt/conda")
 

@@ -36,9 +36,21 @@ struct LineSet {
     }
 };
 
-// Tile index of this work-group along blockIdx.x.  The dispatcher hands
-// work-groups to the eight XCDs in turn (work-group w to XCD w mod 8, w the
-// linear work-group id; w and w + 8 land on one XCD whatever the other grid
+// Work-group coordinates of one pass: a standalone launch takes them from
+// the hardware (hw_blk), a grouped launch -- passes of several grid sizes in
+// one launch -- from its segment table.  gx is the number of tiles along x.
+struct Blk {
+    unsigned x, y, z, gx;
+};
+
+__device__ __forceinline__ Blk hw_blk()
+{
+    return Blk{blockIdx.x, blockIdx.y, blockIdx.z, gridDim.x};
+}
+
+// Tile index of a work-group along x.  The dispatcher hands work-groups to
+// the eight XCDs in turn (work-group w to XCD w mod 8, w the linear
+// work-group id; w and w + 8 land on one XCD whatever the other grid
 // dimensions are), and every XCD has its own L2.  Strided tiles that are
 // neighbours share a 128-byte line at their edges (the half-spectrum rows of
 // Nz/2 + 1 complex values start off line boundaries), so with SWZ each XCD
@@ -46,12 +58,12 @@ struct LineSet {
 // x*q + min(x, r) + [0, q + (x < r)), q = gx / 8, r = gx mod 8, in order.
 // The map is a bijection of [0, gx).
 template <bool SWZ>
-__device__ __forceinline__ unsigned tile_x()
+__device__ __forceinline__ unsigned tile_x(const Blk &bk)
 {
     if constexpr (!SWZ) {
-        return blockIdx.x;
+        return bk.x;
     } else {
-        const unsigned gx = gridDim.x, bx = blockIdx.x;
+        const unsigned gx = bk.gx, bx = bk.x;
         const unsigned q = gx / 8, r = gx % 8, x = bx % 8;
         return x * q + (x < r ? x : r) + bx / 8;
     }
@@ -68,19 +80,24 @@ __device__ __forceinline__ unsigned tile_x()
 // The line plan P and the work-group size WG are template parameters so
 // that an autotuner can pick them per line length (B = WG / (L / P::E)
 // lines per work-group).
+template <int L, class P = Plan<L>, int WG = FFT_WG>
+struct StridedShape {
+    static constexpr int B = WG / (L / P::E), PITCH = Lds<L>::PITCH;
+    static constexpr int LDS = B * PITCH; // complex values of LDS
+};
+
 template <int L, int DIR, class C = double2, class P = Plan<L>,
           int WG = FFT_WG, bool SWZ = true>
-__global__ __launch_bounds__(WG) void k_c2c_strided(
-    const C *in, C *out, LineSet ls, long batch, const C *__restrict__ tw,
-    long rr)
+__device__ __forceinline__ void c2c_strided_body(
+    const Blk &bk, C *lds, const C *in, C *out, LineSet ls, long batch,
+    const C *__restrict__ tw, long rr)
 {
-    constexpr int B = WG / (L / P::E), PITCH = Lds<L>::PITCH;
-    __shared__ C lds[B * PITCH];
+    constexpr int B = StridedShape<L, P, WG>::B, PITCH = Lds<L>::PITCH;
     const int tid = threadIdx.x;
-    in += blockIdx.y * batch + blockIdx.z * rr;
-    out += blockIdx.y * batch + blockIdx.z * rr;
+    in += bk.y * batch + bk.z * rr;
+    out += bk.y * batch + bk.z * rr;
     const int b = tid % B, j = tid / B;
-    const long line = (long)tile_x<SWZ>() * B + b;
+    const long line = (long)tile_x<SWZ>(bk) * B + b;
     const bool act = line < ls.nlines;
     const long base = ls.base(act ? line : 0);
     const C *src = in + base;
@@ -94,6 +111,17 @@ __global__ __launch_bounds__(WG) void k_c2c_strided(
         if (act) stn(dst + n * st, v);
     };
     fft_line<L, DIR, true, true, P>(lds + b * PITCH, j, tw, rd, wr);
+}
+
+template <int L, int DIR, class C = double2, class P = Plan<L>,
+          int WG = FFT_WG, bool SWZ = true>
+__global__ __launch_bounds__(WG) void k_c2c_strided(
+    const C *in, C *out, LineSet ls, long batch, const C *__restrict__ tw,
+    long rr)
+{
+    __shared__ C lds[StridedShape<L, P, WG>::LDS];
+    c2c_strided_body<L, DIR, C, P, WG, SWZ>(hw_blk(), lds, in, out, ls, batch,
+                                            tw, rr);
 }
 
 // Candidate (plan, work-group size) pairs of the strided pass per line
@@ -130,7 +158,7 @@ template <> struct StridedCands<64> {
 };
 template <> struct StridedCands<128> {
     using type = CandList<StridedCand<Plan<128>, 256>, StridedCand<Plan<128>, 256, false>,
-                          StridedCand<PlanT<8, 8, 8, 2>, 256>,
+                          StridedCand<PlanT<16, 16, 8>, 256>,
                           StridedCand<PlanT<4, 4, 4, 4, 2>, 256>, StridedCand<Plan<128>, 128>>;
     static constexpr int x_default = 0;
 };
@@ -171,17 +199,17 @@ auto with_cand(int id, CandList<C0, Cs...>, F &&f)
 // ---------------------------------------------------------------------------
 template <int L, int NQ, bool FWD, class Fac, class P = Plan<L>,
           int WG = FFT_WG, bool SWZ = true>
-__global__ __launch_bounds__(WG) void k_x_fanout(
-    const double2 *in, double2 *out, long qstride, LineSet ls, long batch,
-    const double2 *__restrict__ tw, Fac fac, long rr)
+__device__ __forceinline__ void x_fanout_body(
+    const Blk &bk, double2 *lds, const double2 *in, double2 *out,
+    long qstride, LineSet ls, long batch, const double2 *__restrict__ tw,
+    Fac fac, long rr)
 {
-    constexpr int E = P::E, T = L / E, B = WG / T, PITCH = Lds<L>::PITCH;
-    __shared__ double2 lds[B * PITCH];
+    constexpr int E = P::E, B = StridedShape<L, P, WG>::B, PITCH = Lds<L>::PITCH;
     const int tid = threadIdx.x;
-    in += blockIdx.y * batch + blockIdx.z * rr;
-    out += blockIdx.y * batch + blockIdx.z * rr;
+    in += bk.y * batch + bk.z * rr;
+    out += bk.y * batch + bk.z * rr;
     const int b = tid % B, j = tid / B;
-    const long line = (long)tile_x<SWZ>() * B + b;
+    const long line = (long)tile_x<SWZ>(bk) * B + b;
     const bool act = line < ls.nlines;
     const long base = ls.base(act ? line : 0);
     const double2 *src = in + base;
@@ -206,13 +234,24 @@ __global__ __launch_bounds__(WG) void k_x_fanout(
             __syncthreads(); // LDS reads of the previous transform are done
         double2 *dst = out + q * qstride + base;
         auto rq = [&](int slot, int n) {
-            return fac(q, line, n, spec[FWD ? chain_slot<L, P>(slot) : slot]);
+            return fac(bk, q, line, n, spec[FWD ? chain_slot<L, P>(slot) : slot]);
         };
         auto wq = [&](int, int n, double2 v) {
             if (act) stn(dst + n * st, v);
         };
         fft_line<L, +1, true, true, P>(my, j, tw, rq, wq);
     }
+}
+
+template <int L, int NQ, bool FWD, class Fac, class P = Plan<L>,
+          int WG = FFT_WG, bool SWZ = true>
+__global__ __launch_bounds__(WG) void k_x_fanout(
+    const double2 *in, double2 *out, long qstride, LineSet ls, long batch,
+    const double2 *__restrict__ tw, Fac fac, long rr)
+{
+    __shared__ double2 lds[StridedShape<L, P, WG>::LDS];
+    x_fanout_body<L, NQ, FWD, Fac, P, WG, SWZ>(hw_blk(), lds, in, out, qstride,
+                                               ls, batch, tw, fac, rr);
 }
 
 // Line plans of the derivative fan-out.  (At 256 points, 8 values per
@@ -262,21 +301,22 @@ __device__ __forceinline__ double2 herm_merge(const double2 *__restrict__ A,
 }
 
 // Real rows -> half rows.  Line l = rows (2l, 2l+1) of `in` (nrows rows of
-// L reals, batched over blockIdx.y with `rbatch` / `cbatch` strides).
+// L reals, batched over y with `rbatch` / `cbatch` strides, realizations
+// over z `rin` / `rout` apart).
 template <int L>
-__global__ __launch_bounds__(FFT_WG) void k_z_r2c(
-    const double *in, double2 *out, long nrows, long rbatch, long cbatch,
-    const double2 *__restrict__ tw, long rin, long rout)
+__device__ __forceinline__ void z_r2c_body(
+    const Blk &bk, double2 *lds, const double *in, double2 *out, long nrows,
+    long rbatch, long cbatch, const double2 *__restrict__ tw, long rin,
+    long rout)
 {
     using TL = Tile<L>;
     constexpr int MZ = L / 2 + 1;
-    __shared__ double2 lds[TL::B * TL::PITCH];
     const int tid = threadIdx.x;
-    in += blockIdx.y * rbatch + blockIdx.z * rin;
-    out += blockIdx.y * cbatch + blockIdx.z * rout;
+    in += bk.y * rbatch + bk.z * rin;
+    out += bk.y * cbatch + bk.z * rout;
     const long nlines = nrows / 2;
     const int bc = tid / TL::T;
-    const long line = (long)blockIdx.x * TL::B + bc;
+    const long line = (long)bk.x * TL::B + bc;
     const bool act = line < nlines;
     const double *ra = in + (act ? 2 * line : 0) * L;
     auto rd = [&](int, int n) {
@@ -284,7 +324,7 @@ __global__ __launch_bounds__(FFT_WG) void k_z_r2c(
         return act ? make_double2(u, v) : make_double2(0.0, 0.0);
     };
     fft_line<L, -1, true>(lds + bc * TL::PITCH, tid % TL::T, tw, rd);
-    const long row0 = 2 * (long)blockIdx.x * TL::B;
+    const long row0 = 2 * (long)bk.x * TL::B;
     for (int idx = tid; idx < 2 * TL::B * MZ; idx += FFT_WG) {
         const int r = idx / MZ, k = idx - r * MZ;
         if (row0 + r < nrows)
@@ -293,21 +333,30 @@ __global__ __launch_bounds__(FFT_WG) void k_z_r2c(
     }
 }
 
+template <int L>
+__global__ __launch_bounds__(FFT_WG) void k_z_r2c(
+    const double *in, double2 *out, long nrows, long rbatch, long cbatch,
+    const double2 *__restrict__ tw, long rin, long rout)
+{
+    __shared__ double2 lds[Tile<L>::B * Tile<L>::PITCH];
+    z_r2c_body<L>(hw_blk(), lds, in, out, nrows, rbatch, cbatch, tw, rin, rout);
+}
+
 // Half rows -> real rows (the c2r z pass).
 template <int L>
-__global__ __launch_bounds__(FFT_WG) void k_z_c2r(
-    const double2 *in, double *out, long nrows, long cbatch, long rbatch,
-    const double2 *__restrict__ tw, long rin, long rout)
+__device__ __forceinline__ void z_c2r_body(
+    const Blk &bk, double2 *lds, const double2 *in, double *out, long nrows,
+    long cbatch, long rbatch, const double2 *__restrict__ tw, long rin,
+    long rout)
 {
     using TL = Tile<L>;
     constexpr int MZ = L / 2 + 1;
-    __shared__ double2 lds[TL::B * TL::PITCH];
     const int tid = threadIdx.x;
-    in += blockIdx.y * cbatch + blockIdx.z * rin;
-    out += blockIdx.y * rbatch + blockIdx.z * rout;
+    in += bk.y * cbatch + bk.z * rin;
+    out += bk.y * rbatch + bk.z * rout;
     const long nlines = nrows / 2;
     const int bc = tid / TL::T;
-    const long line = (long)blockIdx.x * TL::B + bc;
+    const long line = (long)bk.x * TL::B + bc;
     const bool act = line < nlines;
     const double2 *A = in + (act ? 2 * line : 0) * MZ;
     auto rd = [&](int, int n) {
@@ -315,7 +364,7 @@ __global__ __launch_bounds__(FFT_WG) void k_z_c2r(
         return act ? z : make_double2(0.0, 0.0);
     };
     fft_line<L, +1, true>(lds + bc * TL::PITCH, tid % TL::T, tw, rd);
-    const long row0 = 2 * (long)blockIdx.x * TL::B;
+    const long row0 = 2 * (long)bk.x * TL::B;
     for (int idx = tid; idx < 2 * TL::B * L; idx += FFT_WG) {
         const int r = idx / L, n = idx - r * L;
         if (row0 + r < nrows) {
@@ -323,6 +372,15 @@ __global__ __launch_bounds__(FFT_WG) void k_z_c2r(
             stn(out + row0 * L + idx, (r & 1) ? z.y : z.x);
         }
     }
+}
+
+template <int L>
+__global__ __launch_bounds__(FFT_WG) void k_z_c2r(
+    const double2 *in, double *out, long nrows, long cbatch, long rbatch,
+    const double2 *__restrict__ tw, long rin, long rout)
+{
+    __shared__ double2 lds[Tile<L>::B * Tile<L>::PITCH];
+    z_c2r_body<L>(hw_blk(), lds, in, out, nrows, cbatch, rbatch, tw, rin, rout);
 }
 
 } // namespace ihg

@@ -24,6 +24,7 @@
 #include "ihg_internal.h"
 #include "mem.h"
 #include "reduce.h"
+#include "rhs_kernels.h"
 
 namespace ihg {
 
@@ -34,510 +35,14 @@ static inline unsigned nblocks(long n, int per_block, unsigned cap)
     return (unsigned)(b > (long)cap ? cap : b);
 }
 
-// ---------------------------------------------------------------------------
-// spectral derivative factors (mk_gradient_squared_and_laplacian,
-// src/toolbox.c:132-139): pre = phi_hat / N; outputs pre * i * k_x,
-// pre * i * k_y, pre * i * k_z (Nyquist-zeroed) and -pre * k^2
-// ---------------------------------------------------------------------------
-struct DerivFac {
-    KGeom kg;
-    double Nd;
-    __device__ double2 operator()(int q, long line, int n, double2 v) const
-    {
-        const int iy = (int)(line / kg.Mz), iz = (int)(line - (long)iy * kg.Mz);
-        const double2 pre = make_double2(v.x / Nd, v.y / Nd);
-        if (q == 3) {
-            const double ks = kg.ksq(n, iy, iz);
-            return make_double2(-(pre.x * ks), -(pre.y * ks));
-        }
-        const double k = (q == 0) ? kg.kxd(n) : ((q == 1) ? kg.kyd(iy) : kg.kzd(iz));
-        return make_double2(-pre.y * k, pre.x * k);
-    }
-};
-
-// capply_filter (src/toolbox.c:616-626): in *= filter; out = in / N.
-// The filtered spectra in *= filter of phi (batch 0) and psi (batch 2) are
-// kept, as mk_ffts_and_filter keeps tmp.phic / tmp.psic
-// (src/toolbox.c:164-171): phi's in keep_phi (and keep_phi2), psi's in
-// keep_psi; null pointers keep nothing.
-struct FilterFac {
-    const double *__restrict__ filt; // Fx[Nx] | Fy[Ny] | Fz[Mz]
-    long plane; // Ny * Mz
-    int Nx, Ny, Mz;
-    double Nd;
-    double2 *keep_phi, *keep_phi2, *keep_psi;
-    long rkeep; // realization stride of keep_phi (ensembles)
-    __device__ double2 operator()(int, long line, int n, double2 v) const
-    {
-        const bool in = line < plane; // padding lines read entry 0
-        const long lc = in ? line : 0;
-        const long i = (long)n * plane + lc;
-        // the mask at (n, j, k) of line j Mz + k: the smallest axis window
-        const int j = (int)(lc / Mz), k = (int)(lc - (long)j * Mz);
-        const double F = fmin(fmin(filt[n], filt[Nx + j]), filt[Nx + Ny + k]);
-        const double2 w = make_double2(v.x * F, v.y * F);
-        if (in && blockIdx.y == 0 && keep_phi) {
-            keep_phi[blockIdx.z * rkeep + i] = w;
-            if (keep_phi2) keep_phi2[i] = w;
-        }
-        if (in && blockIdx.y == 2 && keep_psi)
-            keep_psi[i] = w;
-        return make_double2(w.x / Nd, w.y / Nd);
-    }
-};
-
-// ---------------------------------------------------------------------------
-// z c2r of the four derivative spectra fused with assemble_gradient_squared
-// and mk_rho_and_p (src/toolbox.c:196-243).  Line 2r holds (d_x, d_y) of row
-// r, line 2r+1 holds (d_z, lap).
-// ---------------------------------------------------------------------------
-template <int L, class G>
-__global__ __launch_bounds__(FFT_WG) void k_z_c2r_rho(
-    const double2 *__restrict__ W, long M, const double *__restrict__ f,
-    double *__restrict__ result, G *__restrict__ grad3,
-    double *__restrict__ rho, double *__restrict__ pressure,
-    double *__restrict__ partials, long nrows, long N, double m2,
-    const double2 *__restrict__ tw, Ens e)
-{
-    // realization blockIdx.z of an ensemble
-    W += blockIdx.z * e.w;
-    f += blockIdx.z * e.f;
-    result += blockIdx.z * e.f;
-    partials += blockIdx.z * e.p;
-    if (grad3) grad3 += blockIdx.z * e.g;
-    using TL = Tile<L>;
-    constexpr int MZ = L / 2 + 1;
-    constexpr int RPB = TL::B / 2; // rows per block
-    __shared__ double2 lds[TL::B * TL::PITCH];
-    const int tid = threadIdx.x;
-    const int bc = tid / TL::T;
-    const long line = (long)blockIdx.x * TL::B + bc;
-    const bool act = line < 2 * nrows;
-    const long row = act ? (line >> 1) : 0;
-    const int p = (int)(line & 1);
-    const double2 *A = W + (2 * p) * M + row * MZ;
-    const double2 *Bq = W + (2 * p + 1) * M + row * MZ;
-    auto rd = [&](int, int n) {
-        const double2 z = herm_merge<L>(A, Bq, n); // in bounds for every lane
-        return act ? z : make_double2(0.0, 0.0);
-    };
-    // phi, dphi, psi of the points this thread finishes (K = E / 2 per
-    // thread) are loaded before the transform, so their latency hides
-    // behind it; rows past the end read row 0 (in bounds, unused)
-    constexpr int K = RPB * L / FFT_WG;
-    static_assert(K * FFT_WG == RPB * L, "whole rows per thread sweep");
-    const long row0 = (long)blockIdx.x * RPB;
-    double php[K], dphp[K], psp[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const int idx = tid + k * FFT_WG;
-        const int rr = idx / L, n = idx - rr * L;
-        const long r = row0 + rr;
-        const long i = (r < nrows ? r : 0) * L + n;
-        php[k] = f[i];
-        dphp[k] = f[N + i];
-        psp[k] = f[2 * N + i];
-    }
-    fft_line<L, +1, true>(lds + bc * TL::PITCH, tid % TL::T, tw, rd);
-
-    const long Ntot = 4 * N + 4 * (2 * M) + 1;
-    const double aa = f[Ntot - 1];
-    const double a2 = aa * aa;
-    double srho = 0.0, sp = 0.0;
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const int idx = tid + k * FFT_WG;
-        const int rr = idx / L, n = idx - rr * L;
-        const long r = row0 + rr;
-        if (r >= nrows) continue;
-        const double2 z1 = lds[(2 * rr) * TL::PITCH + Lds<L>::idx(n)];
-        const double2 z2 = lds[(2 * rr + 1) * TL::PITCH + Lds<L>::idx(n)];
-        const double dx = z1.x, dy = z1.y, dz = z2.x, lap = z2.y;
-        double g = dx * dx;
-        g += dy * dy;
-        g += dz * dz;
-        const long i = r * L + n;
-        const double ph = php[k], dph = dphp[k], ps = psp[k];
-        const double t1 = (0.5 - ps) * dph * dph;
-        const double t2 = (0.5 + ps) * g / a2;
-        const double V = m2 * ph * ph / 2.0;
-        const double rv = t1 + t2 + V;
-        const double pv = t1 - t2 / 3.0 - V;
-        result[N + i] = lap;
-        result[3 * N + i] = pv;
-        if (grad3) {
-            grad3[i] = (G)dx;
-            grad3[N + i] = (G)dy;
-            grad3[2 * N + i] = (G)dz;
-        }
-        if (rho) {
-            rho[i] = rv;
-            pressure[i] = pv;
-        }
-        srho += rv;
-        sp += pv;
-    }
-    double *sh = reinterpret_cast<double *>(lds);
-    srho = block_reduce<OpSum>(srho, sh, FFT_WG);
-    sp = block_reduce<OpSum>(sp, sh + 8, FFT_WG);
-    if (tid == 0) {
-        partials[2 * blockIdx.x] = srho;
-        partials[2 * blockIdx.x + 1] = sp;
-    }
-}
-
-// rho_mean, pressure_mean (src/toolbox.c:241-242) and da (src/toolbox.c:87)
+// rho_mean, pressure_mean and da of every realization (one work-group each)
 __global__ __launch_bounds__(RED_WG) void k_means(
     const double *__restrict__ partials, int nparts, long N,
     const double *__restrict__ f, double *__restrict__ result, long Ntot,
     double *__restrict__ dscal, Ens e)
 {
     __shared__ double sh[8];
-    // one work-group per realization
-    partials += blockIdx.x * e.p;
-    f += blockIdx.x * e.f;
-    result += blockIdx.x * e.f;
-    dscal += blockIdx.x * e.s;
-    double s0 = 0.0, s1 = 0.0;
-    for (int i = threadIdx.x; i < nparts; i += RED_WG) {
-        s0 += partials[2 * i];
-        s1 += partials[2 * i + 1];
-    }
-    s0 = block_reduce<OpSum>(s0, sh, RED_WG);
-    s1 = block_reduce<OpSum>(s1, sh, RED_WG);
-    if (threadIdx.x == 0) {
-        const double rm = s0 / N, pm = s1 / N;
-        dscal[S_RHO_MEAN] = rm;
-        dscal[S_P_MEAN] = pm;
-        result[Ntot - 1] = f[Ntot - 1] * sqrt(rm / 3.0);
-    }
-}
-
-// update_phi_psi (src/toolbox.c:282-299); lap and p were parked in
-// result[N..2N) and result[3N..4N) by k_z_c2r_rho
-template <bool NT>
-__global__ __launch_bounds__(RED_WG) void k_update_phi_psi(
-    const double *__restrict__ f, double *__restrict__ result, long N,
-    long Ntot, double m2, const double *__restrict__ dscal, Ens e)
-{
-    f += blockIdx.z * e.f;
-    result += blockIdx.z * e.f;
-    dscal += blockIdx.z * e.s;
-    const double aa = f[Ntot - 1];
-    const double a2 = aa * aa;
-    const double hubble = sqrt(dscal[S_RHO_MEAN] / 3.0);
-    const double h3 = 3.0 * hubble;
-    const double pm = dscal[S_P_MEAN];
-    // pairs of points (16-byte loads and stores; N is even and every block
-    // starts 16-byte aligned)
-    const long stride = (long)gridDim.x * RED_WG;
-    const double2 *F = reinterpret_cast<const double2 *>(f);
-    double2 *Rs = reinterpret_cast<double2 *>(result);
-    const long H = N / 2;
-    for (long i = (long)blockIdx.x * RED_WG + threadIdx.x; i < H; i += stride) {
-        const double2 ph = ld2<NT>(F + i), dph = ld2<NT>(F + H + i),
-                      ps = ld2<NT>(F + 2 * H + i), dps = ld2<NT>(F + 3 * H + i);
-        const double2 lap = ld2<NT>(Rs + H + i), pr = ld2<NT>(Rs + 3 * H + i);
-        st2<NT>(Rs + i, dph);
-        st2<NT>(Rs + H + i, make_double2(
-            (1.0 + 4.0 * ps.x) * lap.x / a2 - (h3 - 4.0 * dps.x) * dph.x -
-                (1.0 + 2.0 * ps.x) * (m2 * ph.x),
-            (1.0 + 4.0 * ps.y) * lap.y / a2 - (h3 - 4.0 * dps.y) * dph.y -
-                (1.0 + 2.0 * ps.y) * (m2 * ph.y)));
-        st2<NT>(Rs + 2 * H + i, dps);
-        st2<NT>(Rs + 3 * H + i, make_double2(
-            0.5 * pr.x + (ps.x - 0.5) * pm - 4.0 * hubble * dps.x,
-            0.5 * pr.y + (ps.y - 0.5) * pm - 4.0 * hubble * dps.y));
-    }
-}
-
-// ---------------------------------------------------------------------------
-// GW sources: z r2c of the six products d_i phi d_j phi (mk_gw_sources,
-// src/toolbox.c:410-424).  A work-group owns B rows: every thread loads its
-// first-stage points of d_x phi, d_y phi, d_z phi once into registers, then
-// the three complex rows (S11, S12), (S13, S22), (S23, S33) of each row are
-// transformed one after the other; component c = 2q + s goes to W + c M.
-// ---------------------------------------------------------------------------
-template <int L, int Q, class C, class Rows>
-__device__ __forceinline__ void sources_pair(C *lds, int tid, const Rows &g,
-                                             bool act, C *__restrict__ W,
-                                             long M, long row0, long nrows,
-                                             const C *__restrict__ tw)
-{
-    using TL = Tile<L>;
-    using R = typename CT<C>::R;
-    constexpr int MZ = L / 2 + 1;
-    const int bc = tid / TL::T;
-    auto rd = [&](int s, int) {
-        R u, v;
-        if constexpr (Q == 0) {
-            u = g.x[s] * g.x[s];
-            v = g.x[s] * g.y[s];
-        } else if constexpr (Q == 1) {
-            u = g.x[s] * g.z[s];
-            v = g.y[s] * g.y[s];
-        } else {
-            u = g.y[s] * g.z[s];
-            v = g.z[s] * g.z[s];
-        }
-        return act ? CT<C>::make(u, v) : CT<C>::make(0, 0);
-    };
-    fft_line<L, -1, true>(lds + bc * TL::PITCH, tid % TL::T, tw, rd);
-    for (int idx = tid; idx < TL::B * 2 * MZ; idx += FFT_WG) {
-        const int lb = idx / (2 * MZ);
-        const int rem = idx - lb * 2 * MZ;
-        const int s = rem / MZ, k = rem - s * MZ;
-        const long r = row0 + lb;
-        if (r < nrows)
-            W[(2 * Q + s) * M + r * MZ + k] =
-                herm_split<L>(lds + lb * TL::PITCH, k, s);
-    }
-    __syncthreads(); // split reads done before the next transform
-}
-
-template <int L, class R>
-struct GradRegs {
-    R x[Plan<L>::E], y[Plan<L>::E], z[Plan<L>::E];
-};
-
-// C = double2: the FP64 path; C = float2: the precision form (gradients
-// stored in FP32, products, FFTs and output in FP32)
-template <int L, class C>
-__global__ __launch_bounds__(FFT_WG) void k_z_r2c_sources(
-    const typename CT<C>::R *__restrict__ g3, C *__restrict__ W, long M,
-    long nrows, long N, const C *__restrict__ tw, long rg, long rw)
-{
-    g3 += blockIdx.z * rg;
-    W += blockIdx.z * rw;
-    using TL = Tile<L>;
-    using R = typename CT<C>::R;
-    __shared__ C lds[TL::B * TL::PITCH];
-    const int tid = threadIdx.x;
-    const int bc = tid / TL::T, j = tid % TL::T;
-    const long row0 = (long)blockIdx.x * TL::B;
-    const long row = row0 + bc;
-    const bool act = row < nrows;
-    const R *gx = g3 + (act ? row : 0) * L;
-    GradRegs<L, R> g;
-#pragma unroll
-    for (int s = 0; s < TL::E; ++s) {
-        const int n = stage1_point<L>(j, s);
-        g.x[s] = gx[n];
-        g.y[s] = gx[N + n];
-        g.z[s] = gx[2 * N + n];
-    }
-    sources_pair<L, 0>(lds, tid, g, act, W, M, row0, nrows, tw);
-    sources_pair<L, 1>(lds, tid, g, act, W, M, row0, nrows, tw);
-    sources_pair<L, 2>(lds, tid, g, act, W, M, row0, nrows, tw);
-}
-
-// ---------------------------------------------------------------------------
-// TT projection (mk_gw_sources_tt, src/toolbox.c:363-397) as two linear forms
-// of the six source components: o0 = sum_c alpha_c S_c, o1 = sum_c beta_c S_c.
-// With f = k / k^2 (full k), t1 = sum_ij k_i f_j S_ij, t2 = tr S, s1 = t1 + t2,
-// s2 = t1 - t2 and the kept axis pair (p, q) = (x, y) if |kz| > eps, (x, z)
-// if |ky| > eps, (y, z) otherwise, the reference computes
-//   o0 = S_pp - 2 f_p (sum_j k_j S_pj) + (f_p k_p s1 + s2) / 2
-//   o1 = S_pq - f_p (sum_j k_j S_qj) - f_q (sum_j k_j S_pj) + f_p k_q s1 / 2
-// alpha_C, beta_C are the derivatives of these with respect to component
-// C = 0..5 = (xx, xy, xz, yy, yz, zz); the k = 0 mode projects to 0.
-// ---------------------------------------------------------------------------
-template <int C, class R>
-__device__ __forceinline__ void tt_coef(R kx, R ky, R kz, R inv_ksq,
-                                        bool b1, bool zero, R &al, R &be)
-{
-    const R fx = kx * inv_ksq, fy = ky * inv_ksq, fz = kz * inv_ksq;
-    const bool b2 = !b1 && fabs((double)ky) > DBL_EPSILON;
-    const bool py = !b1 && !b2; // p = y, else p = x
-    const bool qy = b1;         // q = y, else q = z
-    const R kp = py ? ky : kx, fp = py ? fy : fx;
-    const R kq = qy ? ky : kz, fq = qy ? fy : fz;
-    R t1, t2, rowp, rowq, dpp, dpq;
-    const R O = 0, I = 1, D = 2;
-    if constexpr (C == 0) {
-        t1 = kx * fx; t2 = I; rowp = py ? O : kx; rowq = O;
-        dpp = py ? O : I; dpq = O;
-    } else if constexpr (C == 1) {
-        t1 = D * (kx * fy); t2 = O; rowp = py ? kx : ky;
-        rowq = qy ? kx : O; dpp = O; dpq = (!py && qy) ? I : O;
-    } else if constexpr (C == 2) {
-        t1 = D * (kx * fz); t2 = O; rowp = py ? O : kz;
-        rowq = qy ? O : kx; dpp = O; dpq = (!py && !qy) ? I : O;
-    } else if constexpr (C == 3) {
-        t1 = ky * fy; t2 = I; rowp = py ? ky : O; rowq = qy ? ky : O;
-        dpp = py ? I : O; dpq = O;
-    } else if constexpr (C == 4) {
-        t1 = D * (ky * fz); t2 = O; rowp = py ? kz : O;
-        rowq = qy ? kz : ky; dpp = O; dpq = py ? I : O;
-    } else {
-        t1 = kz * fz; t2 = I; rowp = O; rowq = qy ? O : kz;
-        dpp = O; dpq = O;
-    }
-    const R s1 = t1 + t2, s2 = t1 - t2;
-    const R half = (R)0.5, two = (R)2.0;
-    al = zero ? (R)0 : dpp - two * fp * rowp + half * (fp * kp * s1 + s2);
-    be = zero ? (R)0 : dpq - fp * rowq - fq * rowp + half * fp * kq * s1;
-}
-
-// y transform of source component C (its z and x passes are done) with the
-// TT accumulation folded into the last Stockham stage; `cur` holds the
-// component's first-stage points (loaded one component ahead)
-template <int L, class P, int K, class C>
-__device__ __forceinline__ void tt_component(
-    C *my, int j, const C *__restrict__ tw, const C *cur, const KGeom &kg,
-    int ix, int iz, const typename CT<C>::R *inv_ksq, C *o0, C *o1)
-{
-    using R = typename CT<C>::R;
-    const R kx = (R)kg.kxf(ix), kz = (R)kg.kzf(iz);
-    const bool b1 = fabs(kg.kzf(iz)) > DBL_EPSILON; // branch on the FP64 k
-    if constexpr (K > 0)
-        __syncthreads(); // LDS reads of the previous component are done
-    // Register pressure control (empty asm statements, no instructions):
-    // an opaque copy of the twiddle pointer per component stops the compiler
-    // from loading every stage twiddle once and holding it across all six
-    // transforms, and pinning the accumulators after each component stops it
-    // from deferring the sums (and so keeping all six spectra live).
-    const C *twc = tw;
-    asm volatile("" : "+s"(twc));
-    auto rd = [&](int s, int) { return cur[s]; };
-    auto acc = [&](int s, int n, C v) {
-        // recomputed per component from 1/k^2 alone: an empty asm keeps the
-        // compiler from holding every derived coefficient of every point
-        // live across the six transforms
-        R iv = inv_ksq[s];
-        asm volatile("" : "+v"(iv));
-        const bool zero = (ix == 0) && (iz == 0) && (n == 0);
-        R al, be;
-        tt_coef<K>(kx, (R)kg.kyf(n), kz, iv, b1, zero, al, be);
-        if constexpr (K == 0) {
-            o0[s] = CT<C>::make(al * v.x, al * v.y);
-            o1[s] = CT<C>::make(be * v.x, be * v.y);
-        } else {
-            o0[s] = CT<C>::make(o0[s].x + al * v.x, o0[s].y + al * v.y);
-            o1[s] = CT<C>::make(o1[s].x + be * v.x, o1[s].y + be * v.y);
-        }
-    };
-    fft_line<L, -1, true, true, P>(my, j, twc, rd, acc);
-#pragma unroll
-    for (int s = 0; s < P::E; ++s)
-        asm volatile("" : "+v"(o0[s].x), "+v"(o0[s].y), "+v"(o1[s].x),
-                     "+v"(o1[s].y));
-}
-
-// Line plans of the fused y pass: few values per thread (radix-4 stages),
-// so that the two TT accumulators, 1/k^2 and the transform of every point a
-// thread owns stay in registers.
-template <int L> struct TTPlan;
-template <> struct TTPlan<4>   { using P = PlanT<4, 4>; static constexpr int WG = 256; };
-template <> struct TTPlan<8>   { using P = PlanT<4, 4, 2>; static constexpr int WG = 256; };
-template <> struct TTPlan<16>  { using P = PlanT<4, 4, 4>; static constexpr int WG = 256; };
-template <> struct TTPlan<32>  { using P = PlanT<4, 4, 4, 2>; static constexpr int WG = 256; };
-template <> struct TTPlan<64>  { using P = PlanT<4, 4, 4, 4>; static constexpr int WG = 256; };
-template <> struct TTPlan<128> { using P = PlanT<4, 4, 4, 4, 2>; static constexpr int WG = 512; };
-template <> struct TTPlan<256> { using P = PlanT<4, 4, 4, 4, 4>; static constexpr int WG = 512; };
-template <> struct TTPlan<512> { using P = PlanT<8, 8, 4, 4, 4>; static constexpr int WG = 512; };
-
-// Last pass of the GW sources: y transforms of the six components (z and x
-// passes done), the TT projection, and update_h (src/toolbox.c:311-346):
-//   dh = hdot,  dhdot = -(2 p_mean + k^2/a^2) h - 3 H hdot + 2 S^TT / a^2
-template <int L, int WG, class P, class C>
-__global__ __launch_bounds__(WG) void k_y_tt_update_h(
-    const C *W, long M, const double *__restrict__ f,
-    double *__restrict__ result, long N, long Next, long Ntot, KGeom kg,
-    const double *__restrict__ dscal, LineSet ls, const C *__restrict__ tw,
-    Ens e, long rw)
-{
-    // realization blockIdx.z of an ensemble (rw: its W stride in units of C)
-    W += blockIdx.z * rw;
-    f += blockIdx.z * e.f;
-    result += blockIdx.z * e.f;
-    dscal += blockIdx.z * e.s;
-    using R = typename CT<C>::R;
-    constexpr int E = P::E, T = L / E, B = WG / T, PITCH = Lds<L>::PITCH;
-    __shared__ C lds[B * PITCH];
-    const int tid = threadIdx.x, b = tid % B, j = tid / B;
-    const long line = (long)tile_x<true>() * B + b;
-    const bool act = line < ls.nlines;
-    const long lc = act ? line : 0;
-    const long base = ls.base(lc);
-    const int ix = (int)(lc / kg.Mz), iz = (int)(lc - (long)ix * kg.Mz);
-    const long st = ls.stride;
-    C *my = lds + b * PITCH;
-    R inv[E];
-#pragma unroll
-    for (int s = 0; s < E; ++s) {
-        const int n = last_point<L, P>(j, s);
-        const bool zero = (ix == 0) && (iz == 0) && (n == 0);
-        inv[s] = zero ? (R)0 : (R)(1.0 / kg.ksq(ix, n, iz));
-    }
-    // first-stage points of component c, in bounds for every lane; the
-    // loads of component c + 1 are issued before the transform of c (the
-    // barriers wait on LDS only), so their latency hides behind it
-    auto load = [&](int c, C *buf) {
-        const C *src = W + c * M + base;
-#pragma unroll
-        for (int s = 0; s < E; ++s) {
-            const C v = src[(long)stage1_point<L, P>(j, s) * st];
-            buf[s] = act ? v : CT<C>::make(0, 0);
-        }
-    };
-    C o0[E], o1[E], ba[E], bb[E];
-    load(0, ba);
-    load(1, bb);
-    tt_component<L, P, 0>(my, j, tw, ba, kg, ix, iz, inv, o0, o1);
-    load(2, ba);
-    tt_component<L, P, 1>(my, j, tw, bb, kg, ix, iz, inv, o0, o1);
-    load(3, bb);
-    tt_component<L, P, 2>(my, j, tw, ba, kg, ix, iz, inv, o0, o1);
-    load(4, ba);
-    tt_component<L, P, 3>(my, j, tw, bb, kg, ix, iz, inv, o0, o1);
-    load(5, bb);
-    tt_component<L, P, 4>(my, j, tw, ba, kg, ix, iz, inv, o0, o1);
-    // update_h's inputs (h and hdot at the points this thread ends with;
-    // line 0's points for an idle lane, in bounds): with few points per
-    // thread they are loaded before the last transform, so their latency
-    // hides behind it (with E = 8 the registers are not there)
-    const long Nh1 = 4 * N, Nh2 = Nh1 + Next, Ndh1 = Nh2 + Next,
-               Ndh2 = Ndh1 + Next;
-    const double2 *f2 = reinterpret_cast<const double2 *>(f);
-    constexpr bool EARLY = E <= 4;
-    double2 h1v[E], h2v[E], d1v[E], d2v[E];
-    auto load_h = [&]() {
-#pragma unroll
-        for (int s = 0; s < E; ++s) {
-            const long i = base + last_point<L, P>(j, s) * st;
-            h1v[s] = f2[Nh1 / 2 + i];
-            h2v[s] = f2[Nh2 / 2 + i];
-            d1v[s] = f2[Ndh1 / 2 + i];
-            d2v[s] = f2[Ndh2 / 2 + i];
-        }
-    };
-    if constexpr (EARLY) load_h();
-    tt_component<L, P, 5>(my, j, tw, bb, kg, ix, iz, inv, o0, o1);
-    if constexpr (!EARLY) load_h();
-    if (!act)
-        return;
-    const double aa = f[Ntot - 1];
-    const double a2 = aa * aa;
-    const double hubble = sqrt(dscal[S_RHO_MEAN] / 3.0);
-    const double h3 = 3.0 * hubble;
-    const double pm = dscal[S_P_MEAN];
-    double2 *r2 = reinterpret_cast<double2 *>(result);
-#pragma unroll
-    for (int s = 0; s < E; ++s) {
-        const int n = last_point<L, P>(j, s);
-        const long i = base + n * st;
-        const double ksq = kg.ksq(ix, n, iz);
-        const double2 h1 = h1v[s], h2 = h2v[s], d1 = d1v[s], d2 = d2v[s];
-        r2[Nh1 / 2 + i] = d1;
-        r2[Nh2 / 2 + i] = d2;
-        const double tmp = -(2.0 * pm + ksq / a2);
-        const double s0x = o0[s].x, s0y = o0[s].y, s1x = o1[s].x, s1y = o1[s].y;
-        r2[Ndh1 / 2 + i] = make_double2(tmp * h1.x - h3 * d1.x + 2.0 * s0x / a2,
-                                        tmp * h1.y - h3 * d1.y + 2.0 * s0y / a2);
-        r2[Ndh2 / 2 + i] = make_double2(tmp * h2.x - h3 * d2.x + 2.0 * s1x / a2,
-                                        tmp * h2.y - h3 * d2.y + 2.0 * s1y / a2);
-    }
+    means_body(blockIdx.x, sh, partials, nparts, N, f, result, Ntot, dscal, e);
 }
 
 // ---------------------------------------------------------------------------

@@ -4,7 +4,9 @@
 grid whose RHS kernels run as one launch each (the realization is the third
 grid dimension).  ``run`` steps several ensembles (one per grid size) by
 fixed-step RK4, by default as one HIP graph launch per step and ensemble,
-each on its ensemble's own stream.  ``run_mixed`` is the mixed-size ensemble of 48 x 32^3 + 16 x 64^3 +
+each on its ensemble's own stream; with ``grouped`` every pass of all
+ensembles is one launch (ih_ens_run_rk4_grouped), optionally captured as one
+graph per step.  ``run_mixed`` is the mixed-size ensemble of 48 x 32^3 + 16 x 64^3 +
 4 x 128^3 realizations (independent seeds, amplitudes log-uniform in
 [1e-6, 1e-4], RK4 dt 0.005) with the ensemble mean and variance of the GW
 spectrum per size.  The reference runs one realization per process and has
@@ -105,14 +107,18 @@ class Ensemble:
                          for r in range(self.nreal)])
 
 
-def run(ensembles, nsteps, dt, use_graph=True):
+def run(ensembles, nsteps, dt, use_graph=True, grouped=False):
+    """RK4 steps of several ensembles: per ensemble (each on its own stream,
+    eager or one graph per ensemble and step), or grouped (every pass of all
+    ensembles in one launch, eager or one graph per step)."""
     arr = (ctypes.c_void_p * len(ensembles))(*[e.h for e in ensembles])
-    ih._check(ih.host_lib().ih_ens_run_rk4(arr, len(ensembles), int(nsteps),
-                                           dt, int(bool(use_graph))))
+    H = ih.host_lib()
+    f = H.ih_ens_run_rk4_grouped if grouped else H.ih_ens_run_rk4
+    ih._check(f(arr, len(ensembles), int(nsteps), dt, int(bool(use_graph))))
 
 
 def run_mixed(nsteps=1000, dt=0.005, use_graph=True, seed=1, sizes=MIXED,
-              bins=32, keep_states=False, autotune=False):
+              bins=32, keep_states=False, autotune=False, grouped=False):
     """The mixed-size ensemble; returns per size the GW spectrum mean and
     variance over its realizations and the timings."""
     rng = np.random.default_rng(seed)
@@ -132,9 +138,10 @@ def run_mixed(nsteps=1000, dt=0.005, use_graph=True, seed=1, sizes=MIXED,
         ih.lib().ihg_sync(e.ctx)
     calls0 = sum(e.calls_rhs for e in ens)
     t0 = time.perf_counter()
-    run(ens, nsteps, dt, use_graph)
+    run(ens, nsteps, dt, use_graph, grouped)
     wall = time.perf_counter() - t0
     out = {"wall_s": wall, "steps": nsteps, "dt": dt, "graph": bool(use_graph),
+           "grouped": bool(grouped),
            "rhs_evals": int(sum(e.calls_rhs for e in ens) - calls0),
            "realization_steps_per_s": sum(e.nreal for e in ens) * nsteps / wall,
            "autotuned": {str(k): v for k, v in tuned.items()}, "sizes": {}}

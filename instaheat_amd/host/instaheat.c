@@ -1018,6 +1018,122 @@ int ih_ens_run_rk4(ih_ens *const *es, int n, long nsteps, double dt,
     return rc;
 }
 
+/* one RK4 step of every ensemble of a group (ens_step's calls, each pass of
+   all ensembles one launch); the arrays hold one pointer per ensemble */
+typedef struct {
+    double **y, **k[4], **tmp;
+} ens_arrays;
+
+static int group_step(ihg_group *grp, const ens_arrays *a, unsigned f1,
+                      double t, double dt)
+{
+    int rc = ihg_group_mk_rhs(grp, t, a->y, a->k[0], f1);
+    if (!rc) rc = ihg_group_rk4_stage(grp, a->tmp, a->y, a->k[0], dt, 1);
+    if (!rc) rc = ihg_group_mk_rhs(grp, t + dt / 2.0, a->tmp, a->k[1], 0);
+    if (!rc) rc = ihg_group_rk4_stage(grp, a->tmp, a->y, a->k[1], dt, 1);
+    if (!rc) rc = ihg_group_mk_rhs(grp, t + dt / 2.0, a->tmp, a->k[2], 0);
+    if (!rc) rc = ihg_group_rk4_stage(grp, a->tmp, a->y, a->k[2], dt, 0);
+    if (!rc) rc = ihg_group_mk_rhs(grp, t + dt, a->tmp, a->k[3], 0);
+    if (!rc)
+        rc = ihg_group_rk4_final(grp, a->y, a->k[0], a->k[1], a->k[2], a->k[3],
+                                 dt);
+    return rc;
+}
+
+/* the group over the ensembles' contexts and the pointer tables of their
+   arrays (one allocation, freed by group_arrays_free) */
+static int group_arrays(ih_ens *const *es, int n, ihg_group **grp,
+                        ens_arrays *a, unsigned *f1)
+{
+    ihg_ctx **ctxs = calloc((size_t)n, sizeof *ctxs);
+    double **ptrs = calloc((size_t)n * 6, sizeof *ptrs);
+    if (!ctxs || !ptrs) {
+        free(ctxs);
+        free(ptrs);
+        return IHG_ENOMEM;
+    }
+    a->y = ptrs;
+    for (int j = 0; j < 4; ++j) a->k[j] = ptrs + (size_t)(j + 1) * n;
+    a->tmp = ptrs + (size_t)5 * n;
+    /* the filter flag of ens_step: the group applies it to the ensembles
+       whose grids have the filter on */
+    *f1 = 0u;
+    for (int i = 0; i < n; ++i) {
+        ctxs[i] = es[i]->g;
+        a->y[i] = es[i]->y;
+        for (int j = 0; j < 4; ++j) a->k[j][i] = es[i]->k[j];
+        a->tmp[i] = es[i]->tmp;
+        if (es[i]->cfg.grid.enable_filter) *f1 = IHG_FILTER;
+    }
+    *grp = NULL;
+    int rc = ihg_group_create(ctxs, n, grp);
+    free(ctxs);
+    if (rc) free(ptrs);
+    return rc;
+}
+
+static void group_arrays_free(ihg_group *grp, ens_arrays *a)
+{
+    ihg_group_destroy(grp);
+    free(a->y);
+}
+
+int ih_ens_capture_step_grouped(ih_ens *const *es, int n, double dt,
+                                ihg_graph **out)
+{
+    if (!es || n < 1 || !out) return IHG_EINVAL;
+    *out = NULL;
+    ihg_group *grp;
+    ens_arrays a;
+    unsigned f1;
+    int rc = group_arrays(es, n, &grp, &a, &f1);
+    if (rc) return rc;
+    /* every ensemble's earlier work finished: the group runs on the first
+       ensemble's stream */
+    for (int i = 0; !rc && i < n; ++i) rc = ihg_sync(es[i]->g);
+    if (!rc) rc = ihg_graph_begin(es[0]->g);
+    if (!rc) {
+        const int rc1 = group_step(grp, &a, f1, es[0]->t, dt);
+        const int rc2 = ihg_graph_end(es[0]->g, out);
+        rc = rc1 ? rc1 : rc2;
+        if (rc) {
+            ihg_graph_destroy(*out);
+            *out = NULL;
+        }
+    }
+    group_arrays_free(grp, &a);
+    return rc;
+}
+
+int ih_ens_run_rk4_grouped(ih_ens *const *es, int n, long nsteps, double dt,
+                           int use_graph)
+{
+    if (!es || n < 1 || nsteps < 0) return IHG_EINVAL;
+    ihg_graph *graph = NULL;
+    ihg_group *grp = NULL;
+    ens_arrays a;
+    unsigned f1;
+    int rc = 0;
+    if (use_graph) {
+        rc = ih_ens_capture_step_grouped(es, n, dt, &graph);
+    } else {
+        rc = group_arrays(es, n, &grp, &a, &f1);
+        for (int i = 0; !rc && i < n; ++i) rc = ihg_sync(es[i]->g);
+    }
+    for (long s = 0; !rc && s < nsteps; ++s) {
+        rc = graph ? ihg_graph_launch(graph, es[0]->g)
+                   : group_step(grp, &a, f1, es[0]->t, dt);
+        for (int i = 0; i < n; ++i) {
+            es[i]->t += dt;
+            es[i]->calls += 4UL * es[i]->nreal;
+        }
+    }
+    if (!rc) rc = ihg_sync(es[0]->g);
+    ihg_graph_destroy(graph);
+    if (grp) group_arrays_free(grp, &a);
+    return rc;
+}
+
 int ih_ens_observables(ih_ens *e, int r, double *obs, size_t len)
 {
     if (r < 0 || r >= e->nreal) return IHG_EINVAL;

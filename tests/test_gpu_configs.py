@@ -5,9 +5,10 @@
   after 40 DOPRI853 steps against the FP64 GPU run over the same steps;
 * batched: the mixed ensemble 48 x 32^3 + 16 x 64^3 + 4 x 128^3 (seeds
   1..68, amplitudes log-uniform in [1e-6, 1e-4], RK4 dt 0.005, GW and
-  filter on) for 20 steps in HIP graphs -- one realization per size equal to
-  run_rk4 of that realization alone, bit for bit, and one realization's RHS
-  per size against the FP64 oracle;
+  filter on) for 20 steps in HIP graphs, per size and grouped (one launch
+  per pass for all sizes), the two equal bit for bit -- one realization per
+  size equal to run_rk4 of that realization alone, bit for bit, and one
+  realization's RHS per size against the FP64 oracle;
 * learned: 256^3 FP64 DOPRI853 rtol 1e-10 with the step proposer from a
   state in the error-controlled phase (t > 20) -- its end state after four
   accepted steps against the oracle taking the same steps.
@@ -128,8 +129,9 @@ MIXED = ((32, 48), (64, 16), (128, 4))
 
 @pytest.mark.timeout(600)
 def test_mixed_ensemble_full_size(ih, oracle_mod):
-    """The mixed ensemble at its full size for 20 graph-replayed RK4 steps:
-    the last realization of each size equals run_rk4 of that realization
+    """The mixed ensemble at its full size for 20 graph-replayed RK4 steps,
+    per size and grouped (every pass of all sizes one launch), the two equal
+    bit for bit; the last realization of each size equals run_rk4 of that realization
     alone (its final prepare_and_save filter applied to the ensemble's state
     too), bit for bit; the RHS of that realization's end state matches the
     oracle to 1e-12; the ensemble mean and variance of the GW spectrum are
@@ -150,6 +152,18 @@ def test_mixed_ensemble_full_size(ih, oracle_mod):
         ens.append(e)
     print("mixed ensemble: 20 RK4 steps in graphs", flush=True)
     E.run(ens, steps, dt, use_graph=True)
+    # the grouped form (every pass of the three sizes one launch, one graph
+    # per step) from the same initial data: the same states bit for bit
+    grp = []
+    for n, nreal in MIXED:
+        e = E.Ensemble(n, nreal, kmin=kmin, bins=bins)
+        e.init(seeds[n], amps[n])
+        grp.append(e)
+    print("mixed ensemble: 20 grouped RK4 steps in graphs", flush=True)
+    E.run(grp, steps, dt, use_graph=True, grouped=True)
+    for a, b in zip(ens, grp):
+        assert np.array_equal(a.state(), b.state()), a.n
+        b.close()
     L = 2 * np.pi / kmin
     for e in ens:
         n, r = e.n, e.nreal - 1
