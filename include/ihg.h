@@ -227,6 +227,32 @@ int ihg_ensemble(const ihg_ctx *c, long out[2]);
 int ihg_mk_rhs_ensemble(ihg_ctx *c, double t, double *f, double *result,
                         unsigned flags);
 
+/* RK4 folded into mk_rhs (no counterpart in the reference, whose run_rk4
+ * sweeps the state once more per stage, src/rk4.c:78-102): the kernels that
+ * write the slope k = result also write, for the same elements,
+ *   IHG_RK4_STAGE: out = y + dt k / 2 (half = 1) or y + dt k (half = 0),
+ *                  the next stage's input (out differs from f, result, y);
+ *   IHG_RK4_FINAL: y = y + dt (k1 + 2 k2 + 2 k3 + k) / 6 in place (y differs
+ *                  from f and result); the slope k itself is not stored.
+ * The values equal ihg_mk_rhs_ensemble followed by ihg_rk4_stage /
+ * ihg_rk4_final bit for bit, with one sweep of the state fewer per stage
+ * (on contexts without the fused kernels -- the FP32 GW source, or GW
+ * with y lines outside 16 to 128 points -- it runs exactly those calls).
+ * Arrays span all realizations of an ensemble context, as there.  flags:
+ * IHG_FILTER only. */
+#define IHG_RK4_STAGE 1
+#define IHG_RK4_FINAL 2
+typedef struct ihg_rk4_ep {
+    int mode;                    /* IHG_RK4_STAGE or IHG_RK4_FINAL */
+    int half;                    /* STAGE: dt / 2 (1) or dt (0) */
+    double dt;
+    const double *y;             /* the step's start state */
+    double *out;                 /* STAGE: the next stage's input */
+    const double *k1, *k2, *k3;  /* FINAL: the step's first three slopes */
+} ihg_rk4_ep;
+int ihg_mk_rhs_rk4(ihg_ctx *c, double t, double *f, double *result,
+                   unsigned flags, const ihg_rk4_ep *ep);
+
 /* ---- groups: ensembles of several grid sizes in one launch per pass -----
  * (no counterpart in the reference).  A group holds 1 to 4 ensemble (or
  * single) contexts of cubic grids of 16, 32, 64 or 128 points, FP64.
@@ -250,6 +276,11 @@ long ihg_group_launches(const ihg_group *g);
 /* flags: IHG_FILTER only (applied to the contexts with the filter on) */
 int ihg_group_mk_rhs(ihg_group *g, double t, double *const *f,
                      double *const *result, unsigned flags);
+/* ihg_mk_rhs_rk4 on every context: ep holds one epilogue per context, all
+ * of one mode */
+int ihg_group_mk_rhs_rk4(ihg_group *g, double t, double *const *f,
+                         double *const *result, unsigned flags,
+                         const ihg_rk4_ep *ep);
 int ihg_group_rk4_stage(ihg_group *g, double *const *out, double *const *y,
                         double *const *k, double dt, int half);
 int ihg_group_rk4_final(ihg_group *g, double *const *y, double *const *k1,

@@ -58,6 +58,18 @@ class IhgParams(ctypes.Structure):
     ]
 
 
+class IhgRk4Ep(ctypes.Structure):
+    """ihg_rk4_ep (include/ihg.h): the RK4 epilogue of ihg_mk_rhs_rk4."""
+    _fields_ = [("mode", ctypes.c_int), ("half", ctypes.c_int),
+                ("dt", ctypes.c_double), ("y", ctypes.c_void_p),
+                ("out", ctypes.c_void_p), ("k1", ctypes.c_void_p),
+                ("k2", ctypes.c_void_p), ("k3", ctypes.c_void_p)]
+
+
+IHG_RK4_STAGE = 1
+IHG_RK4_FINAL = 2
+
+
 class IhConfig(ctypes.Structure):
     _fields_ = [
         ("grid", IhgParams),
@@ -171,6 +183,7 @@ IHG_SYMBOLS = [
     "ihg_graph_join", "ihg_graph_launch", "ihg_graph_destroy",
     "ihg_group_create", "ihg_group_destroy", "ihg_group_launches",
     "ihg_group_mk_rhs", "ihg_group_rk4_stage", "ihg_group_rk4_final",
+    "ihg_group_mk_rhs_rk4", "ihg_mk_rhs_rk4",
     "ihg_mlp_nparams", "ihg_mlp_create", "ihg_mlp_destroy",
     "ihg_mlp_get_params", "ihg_mlp_set_params", "ihg_mlp_forward",
     "ihg_mlp_grad", "ihg_mlp_train",
@@ -276,6 +289,10 @@ def lib():
         "ihg_group_rk4_stage": (i, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                     ctypes.POINTER(_vp), d, i]),
         "ihg_group_rk4_final": (i, [_vp] + [ctypes.POINTER(_vp)] * 5 + [d]),
+        "ihg_group_mk_rhs_rk4": (i, [_vp, d, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                     ctypes.c_uint, ctypes.POINTER(IhgRk4Ep)]),
+        "ihg_mk_rhs_rk4": (i, [_vp, d, _vp, _vp, ctypes.c_uint,
+                               ctypes.POINTER(IhgRk4Ep)]),
         "ihg_graph_destroy": (i, [_vp]),
         "ihg_mlp_nparams": (i, []),
         "ihg_mlp_create": (i, [ctypes.POINTER(_vp), _fptr, ctypes.c_ulong]),

@@ -9,6 +9,7 @@
 #include <cstdint>
 
 #include "ihg_internal.h"
+#include "rk4_kernels.h"
 #include "mem.h"
 
 namespace ihg {
@@ -574,6 +575,58 @@ extern "C" int ihg_mk_rhs_ensemble(ihg_ctx *c, double t, double *f,
     if (kept) IHG_TRY(rhs_filter(c, f, nr));
     IHG_TRY(rhs_core(c, f, result, 0, kept, nr));
     if (c->p.enable_gw) IHG_TRY(rhs_gw(c, f, result, nr));
+    return 0;
+}
+
+extern "C" int ihg_mk_rhs_rk4(ihg_ctx *c, double t, double *f, double *result,
+                              unsigned flags, const ihg_rk4_ep *ep)
+{
+    if (!c || !ep) return set_error("NULL argument");
+    IHG_TRY(check_ptr(f, "f"));
+    IHG_TRY(check_ptr(result, "result"));
+    IHG_TRY(check_ptr(ep->y, "ep->y"));
+    if (f == result) return set_error("f and result must differ");
+    if (flags & IHG_OUTPUT)
+        return set_error("ihg_mk_rhs_rk4: no output calls");
+    Rk4Ep d{};
+    d.dt = ep->dt;
+    d.y = ep->y;
+    if (ep->mode == IHG_RK4_STAGE) {
+        IHG_TRY(check_ptr(ep->out, "ep->out"));
+        if (ep->out == f || ep->out == result || ep->out == ep->y)
+            return set_error("ihg_mk_rhs_rk4: out must differ from f, result and y");
+        d.mode = RK4_STAGE;
+        d.half = ep->half ? 1 : 0;
+        d.out = ep->out;
+    } else if (ep->mode == IHG_RK4_FINAL) {
+        IHG_TRY(check_ptr(ep->k1, "ep->k1"));
+        IHG_TRY(check_ptr(ep->k2, "ep->k2"));
+        IHG_TRY(check_ptr(ep->k3, "ep->k3"));
+        if (ep->y == f || ep->y == result)
+            return set_error("ihg_mk_rhs_rk4: y must differ from f and result");
+        if (ep->k1 == result || ep->k2 == result || ep->k3 == result)
+            return set_error("ihg_mk_rhs_rk4: k1..k3 must differ from result");
+        d.mode = RK4_FINAL;
+        d.out = const_cast<double *>(ep->y); // y, in place
+        d.k1 = ep->k1;
+        d.k2 = ep->k2;
+        d.k3 = ep->k3;
+    } else {
+        return set_error("ihg_mk_rhs_rk4: mode %d", ep->mode);
+    }
+    const int nr = c->nreal;
+    const int kept = (flags & IHG_FILTER) && c->p.enable_filter;
+    if (!rk4_fusable(c)) {
+        // the same values from the separate stage kernels
+        IHG_TRY(ihg_mk_rhs_ensemble(c, t, f, result, flags));
+        if (d.mode == RK4_STAGE)
+            return ihg_rk4_stage(c, d.out, d.y, result, d.dt, d.half);
+        return ihg_rk4_final(c, d.out, d.k1, d.k2, d.k3, result, d.dt);
+    }
+    c->calls_rhs += nr;
+    if (kept) IHG_TRY(rhs_filter(c, f, nr));
+    IHG_TRY(rhs_core(c, f, result, 0, kept, nr, &d));
+    if (c->p.enable_gw) IHG_TRY(rhs_gw(c, f, result, nr, &d));
     return 0;
 }
 

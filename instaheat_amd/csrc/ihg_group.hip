@@ -249,6 +249,7 @@ struct FamZC2RRho {
     }
 };
 
+template <bool EP>
 struct FamMeans {
     static constexpr int WG = RED_WG;
     static constexpr bool BY_LEN = false;
@@ -261,17 +262,20 @@ struct FamMeans {
         long Ntot;
         double *dscal;
         Ens e;
+        Rk4Ep ep;
     };
     template <int L> static constexpr int lds() { return 4; } // 8 doubles
     template <int L>
     __device__ static void run(const Blk &bk, double2 *lds, const Args &a)
     {
         // one work-group per realization: x is the realization
-        means_body(bk.x, reinterpret_cast<double *>(lds), a.partials, a.nparts,
-                   a.N, a.f, a.result, a.Ntot, a.dscal, a.e);
+        means_body<EP>(bk.x, reinterpret_cast<double *>(lds), a.partials,
+                       a.nparts, a.N, a.f, a.result, a.Ntot, a.dscal, a.e,
+                       a.ep);
     }
 };
 
+template <bool EP>
 struct FamUpdate {
     static constexpr int WG = RED_WG;
     static constexpr bool BY_LEN = false;
@@ -283,17 +287,19 @@ struct FamUpdate {
         const double *dscal;
         Ens e;
         int nt;
+        Rk4Ep ep;
     };
     template <int L> static constexpr int lds() { return 0; }
     template <int L>
     __device__ static void run(const Blk &bk, double2 *, const Args &a)
     {
         if (a.nt)
-            update_phi_psi_body<true>(bk.x, bk.gx, bk.z, a.f, a.result, a.N,
-                                      a.Ntot, a.m2, a.dscal, a.e);
+            update_phi_psi_body<true, EP>(bk.x, bk.gx, bk.z, a.f, a.result, a.N,
+                                          a.Ntot, a.m2, a.dscal, a.e, a.ep);
         else
-            update_phi_psi_body<false>(bk.x, bk.gx, bk.z, a.f, a.result, a.N,
-                                       a.Ntot, a.m2, a.dscal, a.e);
+            update_phi_psi_body<false, EP>(bk.x, bk.gx, bk.z, a.f, a.result,
+                                           a.N, a.Ntot, a.m2, a.dscal, a.e,
+                                           a.ep);
     }
 };
 
@@ -320,6 +326,7 @@ struct FamSources {
 // point lines with 512 lanes; the plan, and so every value, is the same)
 template <int L> struct GroupTTPlan { using P = typename TTPlan<L>::P; };
 
+template <bool EP>
 struct FamTT {
     static constexpr int WG = FFT_WG;
     static constexpr bool BY_LEN = true;
@@ -335,6 +342,7 @@ struct FamTT {
         const double2 *tw;
         Ens e;
         long rw;
+        Rk4Ep ep;
     };
     template <int L>
     static constexpr int lds()
@@ -344,9 +352,9 @@ struct FamTT {
     template <int L>
     __device__ static void run(const Blk &bk, double2 *lds, const Args &a)
     {
-        y_tt_update_h_body<L, WG, typename GroupTTPlan<L>::P, double2>(
+        y_tt_update_h_body<L, WG, typename GroupTTPlan<L>::P, double2, EP>(
             bk, lds, a.W, a.M, a.f, a.result, a.N, a.Next, a.Ntot, a.kg,
-            a.dscal, a.ls, a.tw, a.e, a.rw);
+            a.dscal, a.ls, a.tw, a.e, a.rw, a.ep);
     }
 };
 
@@ -575,15 +583,13 @@ static int check_arrays(const ihg_group *g, double *const *a, const char *name)
     return 0;
 }
 
-extern "C" int ihg_group_mk_rhs(ihg_group *g, double t, double *const *f,
-                                double *const *result, unsigned flags)
+// mk_rhs of every context, with the RK4 epilogue eps[i] of context i when
+// eps is given; the launches of the three epilogue families go by whether
+// eps is given (all contexts or none)
+template <bool EP>
+static int group_rhs(ihg_group *g, double *const *f, double *const *result,
+                     unsigned flags, const Rk4Ep *eps)
 {
-    (void)t;
-    if (!g) return set_error("NULL group");
-    IHG_TRY(check_arrays(g, f, "f"));
-    IHG_TRY(check_arrays(g, result, "result"));
-    if (flags & IHG_OUTPUT)
-        return set_error("ihg_group_mk_rhs: output calls are per realization");
     const int n = (int)g->c.size();
     std::vector<int> kept(n);
     for (int i = 0; i < n; ++i) {
@@ -640,8 +646,8 @@ extern "C" int ihg_group_mk_rhs(ihg_group *g, double t, double *const *f,
         std::vector<Seg<FamFanout<4, false, DerivFac>>> fk;
         std::vector<Seg<FamStrided<+1>>> yb;
         std::vector<Seg<FamZC2RRho>> zr;
-        std::vector<Seg<FamMeans>> mn;
-        std::vector<Seg<FamUpdate>> up;
+        std::vector<Seg<FamMeans<EP>>> mn;
+        std::vector<Seg<FamUpdate<EP>>> up;
         for (int i = 0; i < n; ++i) {
             ihg_ctx *c = g->c[i];
             const Ens e = ens_of(c);
@@ -680,16 +686,18 @@ extern "C" int ihg_group_mk_rhs(ihg_group *g, double t, double *const *f,
             sr.a = {W, M, f[i], result[i], c->p.enable_gw ? c->grad3 : nullptr,
                     c->partials, nrows, N, m2, c->tw, e};
             zr.push_back(sr);
-            Seg<FamMeans> sm;
+            const Rk4Ep ep = EP ? eps[i] : Rk4Ep{};
+            Seg<FamMeans<EP>> sm;
             sm.L = L;
             sm.grid = dim3(nr, 1, 1);
-            sm.a = {c->partials, (int)nb, N, f[i], result[i], c->Ntot, c->dscal, e};
+            sm.a = {c->partials, (int)nb, N, f[i], result[i], c->Ntot, c->dscal,
+                    e, ep};
             mn.push_back(sm);
-            Seg<FamUpdate> su;
+            Seg<FamUpdate<EP>> su;
             su.L = L;
             su.grid = dim3(upd_blocks(N), 1, nr);
             su.a = {f[i], result[i], N, c->Ntot, m2, c->dscal, e,
-                    4.0 * 8.0 * (double)N * nr >= NT_BYTES ? 1 : 0};
+                    4.0 * 8.0 * (double)N * nr >= NT_BYTES ? 1 : 0, ep};
             up.push_back(su);
         }
         IHG_TRY(glaunch(g, z1, "grouped z r2c"));
@@ -707,7 +715,7 @@ extern "C" int ihg_group_mk_rhs(ihg_group *g, double t, double *const *f,
     {
         std::vector<Seg<FamSources>> sz;
         std::vector<Seg<FamStrided<-1>>> sx;
-        std::vector<Seg<FamTT>> tt;
+        std::vector<Seg<FamTT<EP>>> tt;
         for (int i = 0; i < n; ++i) {
             ihg_ctx *c = g->c[i];
             if (!c->p.enable_gw) continue;
@@ -723,13 +731,13 @@ extern "C" int ihg_group_mk_rhs(ihg_group *g, double t, double *const *f,
             const LineSet ls = y_lines_of(c);
             const int Bt = dispatch_len(L, [&](auto Lc) {
                 constexpr int LL = decltype(Lc)::value;
-                return FamTT::WG / (LL / GroupTTPlan<LL>::P::E);
+                return FamTT<EP>::WG / (LL / GroupTTPlan<LL>::P::E);
             });
-            Seg<FamTT> st;
+            Seg<FamTT<EP>> st;
             st.L = L;
             st.grid = dim3((unsigned)((ls.nlines + Bt - 1) / Bt), 1, nr);
             st.a = {c->W, M, f[i], result[i], N, c->Next, c->Ntot, c->kg,
-                    c->dscal, ls, c->tw, e, e.w};
+                    c->dscal, ls, c->tw, e, e.w, EP ? eps[i] : Rk4Ep{}};
             tt.push_back(st);
         }
         IHG_TRY(glaunch(g, sz, "grouped GW sources"));
@@ -738,6 +746,65 @@ extern "C" int ihg_group_mk_rhs(ihg_group *g, double t, double *const *f,
     }
     for (int i = 0; i < n; ++i) g->c[i]->calls_rhs += g->c[i]->nreal;
     return 0;
+}
+
+extern "C" int ihg_group_mk_rhs(ihg_group *g, double t, double *const *f,
+                                double *const *result, unsigned flags)
+{
+    (void)t;
+    if (!g) return set_error("NULL group");
+    IHG_TRY(check_arrays(g, f, "f"));
+    IHG_TRY(check_arrays(g, result, "result"));
+    if (flags & IHG_OUTPUT)
+        return set_error("ihg_group_mk_rhs: output calls are per realization");
+    return group_rhs<false>(g, f, result, flags, nullptr);
+}
+
+extern "C" int ihg_group_mk_rhs_rk4(ihg_group *g, double t,
+                                    double *const *f, double *const *result,
+                                    unsigned flags, const ihg_rk4_ep *ep)
+{
+    (void)t;
+    if (!g || !ep) return set_error("NULL argument");
+    IHG_TRY(check_arrays(g, f, "f"));
+    IHG_TRY(check_arrays(g, result, "result"));
+    if (flags & IHG_OUTPUT)
+        return set_error("ihg_group_mk_rhs_rk4: no output calls");
+    const int n = (int)g->c.size();
+    std::vector<Rk4Ep> eps(n);
+    for (int i = 0; i < n; ++i) {
+        const ihg_rk4_ep &x = ep[i];
+        if (!x.y) return set_error("NULL ep[%d].y", i);
+        Rk4Ep &d = eps[i];
+        d = Rk4Ep{};
+        d.dt = x.dt;
+        d.y = x.y;
+        if (x.mode != ep[0].mode)
+            return set_error("ihg_group_mk_rhs_rk4: one mode for all contexts");
+        if (x.mode == IHG_RK4_STAGE) {
+            if (!x.out || x.out == f[i] || x.out == result[i] || x.out == x.y)
+                return set_error("ihg_group_mk_rhs_rk4: ep[%d].out must be set "
+                                 "and differ from f, result and y", i);
+            d.mode = RK4_STAGE;
+            d.half = x.half ? 1 : 0;
+            d.out = x.out;
+        } else if (x.mode == IHG_RK4_FINAL) {
+            if (!x.k1 || !x.k2 || !x.k3)
+                return set_error("NULL ep[%d].k1..k3", i);
+            if (x.y == f[i] || x.y == result[i] || x.k1 == result[i] ||
+                x.k2 == result[i] || x.k3 == result[i])
+                return set_error("ihg_group_mk_rhs_rk4: ep[%d]: y and k1..k3 "
+                                 "must differ from f and result", i);
+            d.mode = RK4_FINAL;
+            d.out = const_cast<double *>(x.y);
+            d.k1 = x.k1;
+            d.k2 = x.k2;
+            d.k3 = x.k3;
+        } else {
+            return set_error("ihg_group_mk_rhs_rk4: mode %d", x.mode);
+        }
+    }
+    return group_rhs<true>(g, f, result, flags, eps.data());
 }
 
 extern "C" int ihg_group_rk4_stage(ihg_group *g, double *const *out,

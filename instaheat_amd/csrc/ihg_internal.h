@@ -162,16 +162,23 @@ enum {
     S_NSCAL = 16
 };
 
+struct Rk4Ep;
+
 // nr = 1: one realization (f, result, the work arrays of realization 0);
-// nr = c->nreal: every realization of an ensemble context at once
+// nr = c->nreal: every realization of an ensemble context at once.  ep: the
+// RK4 epilogue of the kernels that write the slope (rk4_fusable contexts)
 int rhs_core(ihg_ctx *c, double *f, double *result, int output, int kept,
-             int nr = 1);
+             int nr = 1, const Rk4Ep *ep = nullptr);
+// whether the RK4 epilogue is compiled for the context's kernels (FP64, and
+// with GW on y lines of 16 to 128 points)
+bool rk4_fusable(const ihg_ctx *c);
 int features(ihg_ctx *c, const double *f, double out[3]);
 int default_candidate_x(int len);
 void candidate_counts(const ihg_ctx *c, int n[2]);
 int autotune(ihg_ctx *c, int reps, int out[2], float ms_out[16]);
 int rhs_filter(ihg_ctx *c, double *f, int nr = 1);
-int rhs_gw(ihg_ctx *c, const double *f, double *result, int nr = 1);
+int rhs_gw(ihg_ctx *c, const double *f, double *result, int nr = 1,
+           const Rk4Ep *ep = nullptr);
 int rhs_output(ihg_ctx *c, const double *f, int kept);
 int initial_psi(ihg_ctx *c, double *field, double *dfield);
 

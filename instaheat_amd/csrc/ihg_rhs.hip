@@ -45,6 +45,59 @@ __global__ __launch_bounds__(RED_WG) void k_means(
     means_body(blockIdx.x, sh, partials, nparts, N, f, result, Ntot, dscal, e);
 }
 
+// the same three kernels with the RK4 epilogue of their slopes
+__global__ __launch_bounds__(RED_WG) void k_means_rk4(
+    const double *__restrict__ partials, int nparts, long N,
+    const double *__restrict__ f, double *__restrict__ result, long Ntot,
+    double *__restrict__ dscal, Ens e, Rk4Ep ep)
+{
+    __shared__ double sh[8];
+    means_body<true>(blockIdx.x, sh, partials, nparts, N, f, result, Ntot,
+                     dscal, e, ep);
+}
+
+template <bool NT>
+__global__ __launch_bounds__(RED_WG) void k_update_phi_psi_rk4(
+    const double *__restrict__ f, double *__restrict__ result, long N,
+    long Ntot, double m2, const double *__restrict__ dscal, Ens e, Rk4Ep ep)
+{
+    update_phi_psi_body<NT, true>(blockIdx.x, gridDim.x, blockIdx.z, f, result,
+                                  N, Ntot, m2, dscal, e, ep);
+}
+
+template <int L, int WG, class P>
+__global__ __launch_bounds__(WG) void k_y_tt_update_h_rk4(
+    const double2 *W, long M, const double *__restrict__ f,
+    double *__restrict__ result, long N, long Next, long Ntot, KGeom kg,
+    const double *__restrict__ dscal, LineSet ls,
+    const double2 *__restrict__ tw, Ens e, long rw, Rk4Ep ep)
+{
+    __shared__ double2 lds[(WG / (L / P::E)) * Lds<L>::PITCH];
+    y_tt_update_h_body<L, WG, P, double2, true>(hw_blk(), lds, W, M, f, result,
+                                                N, Next, Ntot, kg, dscal, ls,
+                                                tw, e, rw, ep);
+}
+
+// the y line lengths whose TT pass has the RK4 epilogue compiled in
+template <class F>
+static int dispatch_rk4_len(int L, F &&f)
+{
+    switch (L) {
+    case 16: return f(std::integral_constant<int, 16>{});
+    case 32: return f(std::integral_constant<int, 32>{});
+    case 64: return f(std::integral_constant<int, 64>{});
+    case 128: return f(std::integral_constant<int, 128>{});
+    default: return set_error("no RK4 epilogue for %d-point lines", L);
+    }
+}
+
+bool rk4_fusable(const ihg_ctx *c)
+{
+    if (!c->p.enable_gw) return true;
+    const int L = c->Ny;
+    return !c->p.gw_fp32 && (L == 16 || L == 32 || L == 64 || L == 128);
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -56,7 +109,7 @@ static LineSet x_lineset(const ihg_ctx *c)
 }
 
 int rhs_core(ihg_ctx *c, double *f, double *result, int output, int kept,
-             int nr)
+             int nr, const Rk4Ep *ep)
 {
     const long N = c->N, M = c->M;
     const double Nd = (double)N;
@@ -125,20 +178,34 @@ int rhs_core(ihg_ctx *c, double *f, double *result, int output, int kept,
         });
     }));
     IHG_TRY(timed(c, F_MEANS, 16.0 * nparts * nr, [&]() {
-        hipLaunchKernelGGL(k_means, dim3(nr), dim3(RED_WG), 0, c->stream,
-                           c->partials, nparts, N, f, result, c->Ntot,
-                           c->dscal, e);
+        if (ep)
+            hipLaunchKernelGGL(k_means_rk4, dim3(nr), dim3(RED_WG), 0,
+                               c->stream, c->partials, nparts, N, f, result,
+                               c->Ntot, c->dscal, e, *ep);
+        else
+            hipLaunchKernelGGL(k_means, dim3(nr), dim3(RED_WG), 0, c->stream,
+                               c->partials, nparts, N, f, result, c->Ntot,
+                               c->dscal, e);
         return hip_check(hipGetLastError(), "k_means");
     }));
-    return timed(c, F_UPDATE, 10.0 * 8.0 * N * nr, [&]() {
+    // the epilogue reads y (and k1..k3) and writes the stage output beside
+    // the slope: 2 (stage) or 4 (final, no slope written) more blocks
+    const double ep_blocks = !ep ? 0.0 : (ep->mode == RK4_FINAL ? 4.0 : 8.0);
+    return timed(c, F_UPDATE, (10.0 + ep_blocks) * 8.0 * N * nr, [&]() {
         // the streaming hint when the four blocks of all realizations
         // outgrow the caches
         const bool nt = 4.0 * 8.0 * (double)N * nr >= NT_BYTES;
-        hipLaunchKernelGGL((nt ? k_update_phi_psi<true>
-                               : k_update_phi_psi<false>),
-                           dim3(nblocks(N, RED_WG * 4, 65536), 1, nr),
-                           dim3(RED_WG), 0, c->stream, f, result, N, c->Ntot,
-                           m2, c->dscal, e);
+        const dim3 grid(nblocks(N, RED_WG * 4, 65536), 1, nr);
+        if (ep)
+            hipLaunchKernelGGL((nt ? k_update_phi_psi_rk4<true>
+                                   : k_update_phi_psi_rk4<false>),
+                               grid, dim3(RED_WG), 0, c->stream, f, result, N,
+                               c->Ntot, m2, c->dscal, e, *ep);
+        else
+            hipLaunchKernelGGL((nt ? k_update_phi_psi<true>
+                                   : k_update_phi_psi<false>),
+                               grid, dim3(RED_WG), 0, c->stream, f, result, N,
+                               c->Ntot, m2, c->dscal, e);
         return hip_check(hipGetLastError(), "k_update_phi_psi");
     });
 }
@@ -149,7 +216,7 @@ int rhs_core(ihg_ctx *c, double *f, double *result, int output, int kept,
 template <class C>
 static int rhs_gw_t(ihg_ctx *c, const typename CT<C>::R *g3, C *Wk,
                     const C *tw, const double *f, double *result, int nr,
-                    long rg, long rw)
+                    long rg, long rw, const Rk4Ep *ep)
 {
     const long N = c->N, M = c->M;
     const long nrows = (long)c->Nx * c->Ny;
@@ -172,6 +239,26 @@ static int rhs_gw_t(ihg_ctx *c, const typename CT<C>::R *g3, C *Wk,
         IHG_TRY(x_pass(c, Wk, -1, 6, M, nr, rw));
     // y pass of the six sources, TT projection and update_h: six sources
     // and h1, h2, dh1, dh2 in; the four h blocks of result out
+    if constexpr (std::is_same<C, double2>::value) {
+        if (ep)
+            return dispatch_rk4_len(c->Ny, [&](auto Lc) {
+                constexpr int L = decltype(Lc)::value;
+                using TP = TTPlan<L>;
+                constexpr int B = TP::WG / (L / TP::P::E);
+                const LineSet ls{(long)c->Nx * c->Mz, c->Mz, (long)c->Ny * c->Mz, c->Mz};
+                const unsigned nb = (unsigned)((ls.nlines + B - 1) / B);
+                const double epb = ep->mode == RK4_FINAL ? 8.0 : 16.0;
+                return timed(c, F_TT_UPDATE_H, (6.0 * 16.0 + (8.0 + epb) * 16.0) * M * nr, [&]() {
+                    hipLaunchKernelGGL((k_y_tt_update_h_rk4<L, TP::WG, typename TP::P>),
+                                       dim3(nb, 1, nr), dim3(TP::WG), 0, c->stream, Wk,
+                                       M, f, result, N, c->Next, c->Ntot, c->kg,
+                                       c->dscal, ls, tw, e, rw, *ep);
+                    return hip_check(hipGetLastError(), "k_y_tt_update_h_rk4");
+                });
+            });
+    } else if (ep) {
+        return set_error("no RK4 epilogue for the FP32 GW source");
+    }
     return dispatch_len(c->Ny, [&](auto Lc) {
         constexpr int L = decltype(Lc)::value;
         using TP = TTPlan<L>;
@@ -189,7 +276,8 @@ static int rhs_gw_t(ihg_ctx *c, const typename CT<C>::R *g3, C *Wk,
     });
 }
 
-int rhs_gw(ihg_ctx *c, const double *f, double *result, int nr)
+int rhs_gw(ihg_ctx *c, const double *f, double *result, int nr,
+           const Rk4Ep *ep)
 {
     const Ens e = nr > 1 ? c->ens : Ens{};
     if (c->p.gw_fp32)
@@ -197,9 +285,9 @@ int rhs_gw(ihg_ctx *c, const double *f, double *result, int nr)
         // in grad3 (the same element strides per realization)
         return rhs_gw_t<float2>(c, reinterpret_cast<const float *>(c->grad3),
                                 reinterpret_cast<float2 *>(c->W), c->twf, f,
-                                result, nr, e.g, e.w);
+                                result, nr, e.g, e.w, ep);
     return rhs_gw_t<double2>(c, c->grad3, c->W, c->tw, f, result, nr, e.g,
-                             e.w);
+                             e.w, ep);
 }
 
 // mk_ffts_and_filter's filter branch (src/toolbox.c:164-171): phi, dphi,

@@ -14,6 +14,7 @@
 #include "ihg_internal.h"
 #include "mem.h"
 #include "reduce.h"
+#include "rk4_kernels.h"
 
 namespace ihg {
 
@@ -185,10 +186,13 @@ __global__ __launch_bounds__(FFT_WG) void k_z_c2r_rho(
 
 // rho_mean, pressure_mean (src/toolbox.c:241-242) and da (src/toolbox.c:87)
 // (one work-group per realization r; sh: 8 doubles of LDS)
+// EP: the RK4 epilogue of the scale factor's slope
+template <bool EP = false>
 __device__ __forceinline__ void means_body(
     unsigned r, double *sh, const double *__restrict__ partials, int nparts,
     long N, const double *__restrict__ f, double *__restrict__ result,
-    long Ntot, double *__restrict__ dscal, const Ens &e)
+    long Ntot, double *__restrict__ dscal, const Ens &e,
+    const Rk4Ep &ep = Rk4Ep{})
 {
     partials += r * e.p;
     f += r * e.f;
@@ -205,22 +209,28 @@ __device__ __forceinline__ void means_body(
         const double rm = s0 / N, pm = s1 / N;
         dscal[S_RHO_MEAN] = rm;
         dscal[S_P_MEAN] = pm;
-        result[Ntot - 1] = f[Ntot - 1] * sqrt(rm / 3.0);
+        const double da = f[Ntot - 1] * sqrt(rm / 3.0);
+        if constexpr (EP)
+            rk4_ep_one(ep.shifted(r * e.f), Ntot - 1, da, result + Ntot - 1);
+        else
+            result[Ntot - 1] = da;
     }
 }
 
 // update_phi_psi (src/toolbox.c:282-299); lap and p were parked in
 // result[N..2N) and result[3N..4N) by k_z_c2r_rho
-// (work-group bx of gx on realization z)
-template <bool NT>
+// (work-group bx of gx on realization z; EP: the RK4 epilogue of the four
+// blocks' slopes)
+template <bool NT, bool EP = false>
 __device__ __forceinline__ void update_phi_psi_body(
     unsigned bx, unsigned gx, unsigned z, const double *__restrict__ f,
     double *__restrict__ result, long N, long Ntot, double m2,
-    const double *__restrict__ dscal, const Ens &e)
+    const double *__restrict__ dscal, const Ens &e, const Rk4Ep &ep0 = Rk4Ep{})
 {
     f += z * e.f;
     result += z * e.f;
     dscal += z * e.s;
+    [[maybe_unused]] const Rk4Ep ep = EP ? ep0.shifted(z * e.f) : ep0;
     const double aa = f[Ntot - 1];
     const double a2 = aa * aa;
     const double hubble = sqrt(dscal[S_RHO_MEAN] / 3.0);
@@ -236,16 +246,25 @@ __device__ __forceinline__ void update_phi_psi_body(
         const double2 ph = ld2<NT>(F + i), dph = ld2<NT>(F + H + i),
                       ps = ld2<NT>(F + 2 * H + i), dps = ld2<NT>(F + 3 * H + i);
         const double2 lap = ld2<NT>(Rs + H + i), pr = ld2<NT>(Rs + 3 * H + i);
-        st2<NT>(Rs + i, dph);
-        st2<NT>(Rs + H + i, make_double2(
+        const double2 r1 = make_double2(
             (1.0 + 4.0 * ps.x) * lap.x / a2 - (h3 - 4.0 * dps.x) * dph.x -
                 (1.0 + 2.0 * ps.x) * (m2 * ph.x),
             (1.0 + 4.0 * ps.y) * lap.y / a2 - (h3 - 4.0 * dps.y) * dph.y -
-                (1.0 + 2.0 * ps.y) * (m2 * ph.y)));
-        st2<NT>(Rs + 2 * H + i, dps);
-        st2<NT>(Rs + 3 * H + i, make_double2(
+                (1.0 + 2.0 * ps.y) * (m2 * ph.y));
+        const double2 r3 = make_double2(
             0.5 * pr.x + (ps.x - 0.5) * pm - 4.0 * hubble * dps.x,
-            0.5 * pr.y + (ps.y - 0.5) * pm - 4.0 * hubble * dps.y));
+            0.5 * pr.y + (ps.y - 0.5) * pm - 4.0 * hubble * dps.y);
+        if constexpr (EP) {
+            rk4_ep_pair<NT>(ep, i, dph, Rs + i);
+            rk4_ep_pair<NT>(ep, H + i, r1, Rs + H + i);
+            rk4_ep_pair<NT>(ep, 2 * H + i, dps, Rs + 2 * H + i);
+            rk4_ep_pair<NT>(ep, 3 * H + i, r3, Rs + 3 * H + i);
+        } else {
+            st2<NT>(Rs + i, dph);
+            st2<NT>(Rs + H + i, r1);
+            st2<NT>(Rs + 2 * H + i, dps);
+            st2<NT>(Rs + 3 * H + i, r3);
+        }
     }
 }
 
@@ -456,12 +475,13 @@ template <> struct TTPlan<512> { using P = PlanT<8, 8, 4, 4, 4>; static constexp
 // Last pass of the GW sources: y transforms of the six components (z and x
 // passes done), the TT projection, and update_h (src/toolbox.c:311-346):
 //   dh = hdot,  dhdot = -(2 p_mean + k^2/a^2) h - 3 H hdot + 2 S^TT / a^2
-template <int L, int WG, class P, class C>
+// (EP: the RK4 epilogue of the four h blocks' slopes)
+template <int L, int WG, class P, class C, bool EP = false>
 __device__ __forceinline__ void y_tt_update_h_body(
     const Blk &bk, C *lds, const C *W, long M, const double *__restrict__ f,
     double *__restrict__ result, long N, long Next, long Ntot,
     const KGeom &kg, const double *__restrict__ dscal, const LineSet &ls,
-    const C *__restrict__ tw, const Ens &e, long rw)
+    const C *__restrict__ tw, const Ens &e, long rw, const Rk4Ep &ep0 = Rk4Ep{})
 {
     // realization z of an ensemble (rw: its W stride in units of C)
     W += bk.z * rw;
@@ -544,14 +564,26 @@ __device__ __forceinline__ void y_tt_update_h_body(
         const long i = base + n * st;
         const double ksq = kg.ksq(ix, n, iz);
         const double2 h1 = h1v[s], h2 = h2v[s], d1 = d1v[s], d2 = d2v[s];
-        r2[Nh1 / 2 + i] = d1;
-        r2[Nh2 / 2 + i] = d2;
+        if constexpr (!EP) {
+            r2[Nh1 / 2 + i] = d1;
+            r2[Nh2 / 2 + i] = d2;
+        }
         const double tmp = -(2.0 * pm + ksq / a2);
         const double s0x = o0[s].x, s0y = o0[s].y, s1x = o1[s].x, s1y = o1[s].y;
-        r2[Ndh1 / 2 + i] = make_double2(tmp * h1.x - h3 * d1.x + 2.0 * s0x / a2,
+        const double2 v1 = make_double2(tmp * h1.x - h3 * d1.x + 2.0 * s0x / a2,
                                         tmp * h1.y - h3 * d1.y + 2.0 * s0y / a2);
-        r2[Ndh2 / 2 + i] = make_double2(tmp * h2.x - h3 * d2.x + 2.0 * s1x / a2,
+        const double2 v2 = make_double2(tmp * h2.x - h3 * d2.x + 2.0 * s1x / a2,
                                         tmp * h2.y - h3 * d2.y + 2.0 * s1y / a2);
+        if constexpr (EP) {
+            const Rk4Ep ep = ep0.shifted(bk.z * e.f);
+            rk4_ep_pair<false>(ep, Nh1 / 2 + i, d1, r2 + Nh1 / 2 + i);
+            rk4_ep_pair<false>(ep, Nh2 / 2 + i, d2, r2 + Nh2 / 2 + i);
+            rk4_ep_pair<false>(ep, Ndh1 / 2 + i, v1, r2 + Ndh1 / 2 + i);
+            rk4_ep_pair<false>(ep, Ndh2 / 2 + i, v2, r2 + Ndh2 / 2 + i);
+        } else {
+            r2[Ndh1 / 2 + i] = v1;
+            r2[Ndh2 / 2 + i] = v2;
+        }
     }
 }
 

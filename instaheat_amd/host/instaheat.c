@@ -866,7 +866,7 @@ struct ih_ens {
     int nreal;
     long stride;            /* doubles between realizations' states */
     size_t N, M, Next, Ntot;
-    double *y, *k[4], *tmp; /* nreal * stride doubles each, device */
+    double *y, *k[4], *tmp, *tmp2; /* nreal * stride doubles each, device */
     double t;
     unsigned long calls;    /* RHS evaluations of all realizations */
 };
@@ -897,6 +897,7 @@ int ih_ens_create(const ih_config *cfg, int nreal, ih_ens **out)
     rc = ihg_malloc(e->g, &e->y, n);
     for (int i = 0; !rc && i < 4; ++i) rc = ihg_malloc(e->g, &e->k[i], n);
     if (!rc) rc = ihg_malloc(e->g, &e->tmp, n);
+    if (!rc) rc = ihg_malloc(e->g, &e->tmp2, n);
     if (rc) {
         ih_ens_destroy(e);
         return rc;
@@ -913,6 +914,7 @@ void ih_ens_destroy(ih_ens *e)
         ihg_free(e->g, e->y);
         for (int i = 0; i < 4; ++i) ihg_free(e->g, e->k[i]);
         ihg_free(e->g, e->tmp);
+        ihg_free(e->g, e->tmp2);
         ihg_destroy(e->g);
     }
     free(e);
@@ -945,19 +947,24 @@ int ih_ens_init(ih_ens *e, const uint64_t *seeds, const double *amplitudes,
 
 /* one step of run_rk4 (src/rk4.c:78-102) for every realization; the filter
  * acts at the first stage, as there */
+/* One RK4 step of run_rk4 (src/rk4.c:78-102) on every realization: the
+   stage sums are folded into the RHS kernels that write the slopes
+   (ihg_mk_rhs_rk4), the stage inputs alternating between tmp and tmp2; the
+   values are those of mk_rhs + the separate stage sweeps, bit for bit */
 static int ens_step(ih_ens *e, double t, double dt)
 {
     ihg_ctx *g = e->g;
     const unsigned f1 = e->cfg.grid.enable_filter ? IHG_FILTER : 0u;
-    int rc = ihg_mk_rhs_ensemble(g, t, e->y, e->k[0], f1);
-    if (!rc) rc = ihg_rk4_stage(g, e->tmp, e->y, e->k[0], dt, 1);
-    if (!rc) rc = ihg_mk_rhs_ensemble(g, t + dt / 2.0, e->tmp, e->k[1], 0);
-    if (!rc) rc = ihg_rk4_stage(g, e->tmp, e->y, e->k[1], dt, 1);
-    if (!rc) rc = ihg_mk_rhs_ensemble(g, t + dt / 2.0, e->tmp, e->k[2], 0);
-    if (!rc) rc = ihg_rk4_stage(g, e->tmp, e->y, e->k[2], dt, 0);
-    if (!rc) rc = ihg_mk_rhs_ensemble(g, t + dt, e->tmp, e->k[3], 0);
-    if (!rc)
-        rc = ihg_rk4_final(g, e->y, e->k[0], e->k[1], e->k[2], e->k[3], dt);
+    ihg_rk4_ep st = {IHG_RK4_STAGE, 1, dt, e->y, e->tmp, NULL, NULL, NULL};
+    int rc = ihg_mk_rhs_rk4(g, t, e->y, e->k[0], f1, &st);
+    st.out = e->tmp2;
+    if (!rc) rc = ihg_mk_rhs_rk4(g, t + dt / 2.0, e->tmp, e->k[1], 0, &st);
+    st.half = 0;
+    st.out = e->tmp;
+    if (!rc) rc = ihg_mk_rhs_rk4(g, t + dt / 2.0, e->tmp2, e->k[2], 0, &st);
+    const ihg_rk4_ep fin = {IHG_RK4_FINAL, 0, dt, e->y, NULL,
+                            e->k[0], e->k[1], e->k[2]};
+    if (!rc) rc = ihg_mk_rhs_rk4(g, t + dt, e->tmp, e->k[3], 0, &fin);
     return rc;
 }
 
@@ -1021,22 +1028,28 @@ int ih_ens_run_rk4(ih_ens *const *es, int n, long nsteps, double dt,
 /* one RK4 step of every ensemble of a group (ens_step's calls, each pass of
    all ensembles one launch); the arrays hold one pointer per ensemble */
 typedef struct {
-    double **y, **k[4], **tmp;
+    double **y, **k[4], **tmp, **tmp2;
+    ihg_rk4_ep *ep; /* one per ensemble */
 } ens_arrays;
 
-static int group_step(ihg_group *grp, const ens_arrays *a, unsigned f1,
+static int group_step(ihg_group *grp, const ens_arrays *a, int n, unsigned f1,
                       double t, double dt)
 {
-    int rc = ihg_group_mk_rhs(grp, t, a->y, a->k[0], f1);
-    if (!rc) rc = ihg_group_rk4_stage(grp, a->tmp, a->y, a->k[0], dt, 1);
-    if (!rc) rc = ihg_group_mk_rhs(grp, t + dt / 2.0, a->tmp, a->k[1], 0);
-    if (!rc) rc = ihg_group_rk4_stage(grp, a->tmp, a->y, a->k[1], dt, 1);
-    if (!rc) rc = ihg_group_mk_rhs(grp, t + dt / 2.0, a->tmp, a->k[2], 0);
-    if (!rc) rc = ihg_group_rk4_stage(grp, a->tmp, a->y, a->k[2], dt, 0);
-    if (!rc) rc = ihg_group_mk_rhs(grp, t + dt, a->tmp, a->k[3], 0);
-    if (!rc)
-        rc = ihg_group_rk4_final(grp, a->y, a->k[0], a->k[1], a->k[2], a->k[3],
-                                 dt);
+    for (int i = 0; i < n; ++i)
+        a->ep[i] = (ihg_rk4_ep){IHG_RK4_STAGE, 1, dt, a->y[i], a->tmp[i],
+                                NULL, NULL, NULL};
+    int rc = ihg_group_mk_rhs_rk4(grp, t, a->y, a->k[0], f1, a->ep);
+    for (int i = 0; i < n; ++i) a->ep[i].out = a->tmp2[i];
+    if (!rc) rc = ihg_group_mk_rhs_rk4(grp, t + dt / 2.0, a->tmp, a->k[1], 0, a->ep);
+    for (int i = 0; i < n; ++i) {
+        a->ep[i].half = 0;
+        a->ep[i].out = a->tmp[i];
+    }
+    if (!rc) rc = ihg_group_mk_rhs_rk4(grp, t + dt / 2.0, a->tmp2, a->k[2], 0, a->ep);
+    for (int i = 0; i < n; ++i)
+        a->ep[i] = (ihg_rk4_ep){IHG_RK4_FINAL, 0, dt, a->y[i], NULL,
+                                a->k[0][i], a->k[1][i], a->k[2][i]};
+    if (!rc) rc = ihg_group_mk_rhs_rk4(grp, t + dt, a->tmp, a->k[3], 0, a->ep);
     return rc;
 }
 
@@ -1046,15 +1059,19 @@ static int group_arrays(ih_ens *const *es, int n, ihg_group **grp,
                         ens_arrays *a, unsigned *f1)
 {
     ihg_ctx **ctxs = calloc((size_t)n, sizeof *ctxs);
-    double **ptrs = calloc((size_t)n * 6, sizeof *ptrs);
-    if (!ctxs || !ptrs) {
+    double **ptrs = calloc((size_t)n * 7, sizeof *ptrs);
+    ihg_rk4_ep *eps = calloc((size_t)n, sizeof *eps);
+    if (!ctxs || !ptrs || !eps) {
         free(ctxs);
         free(ptrs);
+        free(eps);
         return IHG_ENOMEM;
     }
     a->y = ptrs;
     for (int j = 0; j < 4; ++j) a->k[j] = ptrs + (size_t)(j + 1) * n;
     a->tmp = ptrs + (size_t)5 * n;
+    a->tmp2 = ptrs + (size_t)6 * n;
+    a->ep = eps;
     /* the filter flag of ens_step: the group applies it to the ensembles
        whose grids have the filter on */
     *f1 = 0u;
@@ -1063,12 +1080,16 @@ static int group_arrays(ih_ens *const *es, int n, ihg_group **grp,
         a->y[i] = es[i]->y;
         for (int j = 0; j < 4; ++j) a->k[j][i] = es[i]->k[j];
         a->tmp[i] = es[i]->tmp;
+        a->tmp2[i] = es[i]->tmp2;
         if (es[i]->cfg.grid.enable_filter) *f1 = IHG_FILTER;
     }
     *grp = NULL;
     int rc = ihg_group_create(ctxs, n, grp);
     free(ctxs);
-    if (rc) free(ptrs);
+    if (rc) {
+        free(ptrs);
+        free(eps);
+    }
     return rc;
 }
 
@@ -1076,6 +1097,7 @@ static void group_arrays_free(ihg_group *grp, ens_arrays *a)
 {
     ihg_group_destroy(grp);
     free(a->y);
+    free(a->ep);
 }
 
 int ih_ens_capture_step_grouped(ih_ens *const *es, int n, double dt,
@@ -1093,7 +1115,7 @@ int ih_ens_capture_step_grouped(ih_ens *const *es, int n, double dt,
     for (int i = 0; !rc && i < n; ++i) rc = ihg_sync(es[i]->g);
     if (!rc) rc = ihg_graph_begin(es[0]->g);
     if (!rc) {
-        const int rc1 = group_step(grp, &a, f1, es[0]->t, dt);
+        const int rc1 = group_step(grp, &a, n, f1, es[0]->t, dt);
         const int rc2 = ihg_graph_end(es[0]->g, out);
         rc = rc1 ? rc1 : rc2;
         if (rc) {
@@ -1122,7 +1144,7 @@ int ih_ens_run_rk4_grouped(ih_ens *const *es, int n, long nsteps, double dt,
     }
     for (long s = 0; !rc && s < nsteps; ++s) {
         rc = graph ? ihg_graph_launch(graph, es[0]->g)
-                   : group_step(grp, &a, f1, es[0]->t, dt);
+                   : group_step(grp, &a, n, f1, es[0]->t, dt);
         for (int i = 0; i < n; ++i) {
             es[i]->t += dt;
             es[i]->calls += 4UL * es[i]->nreal;

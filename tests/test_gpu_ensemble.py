@@ -7,7 +7,7 @@ import ctypes
 import numpy as np
 import pytest
 
-from tests.states import BOX_CONFIG, make_state
+from tests.states import BOX_CONFIG, make_state, sizes
 
 pytestmark = pytest.mark.gpu
 
@@ -68,12 +68,14 @@ def test_ensemble_rhs_equals_single_realizations(ih, shape, gw, flags, fp32):
     L.ihg_destroy(ctx)
 
 
-def test_ensemble_rk4_equals_run_rk4(ih, E):
-    """Realization r of an ensemble stepped by RK4 equals run_rk4 of one
-    simulation from the same initial data, bit for bit.  (Filter off on both
+@pytest.mark.parametrize("gw", [True, False])
+def test_ensemble_rk4_equals_run_rk4(ih, E, gw):
+    """Realization r of an ensemble stepped by RK4 (the stage sums folded
+    into the RHS kernels) equals run_rk4 of one simulation from the same
+    initial data (separate stage sweeps), bit for bit.  (Filter off on both
     sides: run_rk4's final prepare_and_save filters the state once more.)"""
     n, nreal, steps, dt = 16, 3, 6, 0.01
-    e = E.Ensemble(n, nreal, kmin=0.1, bins=16, filt=False)
+    e = E.Ensemble(n, nreal, kmin=0.1, bins=16, filt=False, gw=gw)
     seeds, amps = [7, 8, 9], [1e-5, 3e-5, 1e-4]
     e.init(seeds, amps)
     E.run([e], steps, dt, use_graph=False)
@@ -81,7 +83,7 @@ def test_ensemble_rk4_equals_run_rk4(ih, E):
     L = 2 * np.pi / 0.1
     for r in range(nreal):
         # max_steps rather than tf, so that no last step is shortened
-        sim = ih.Sim((n, n, n), bounds=((0.0, L),) * 3, mass=1.0, gw=True,
+        sim = ih.Sim((n, n, n), bounds=((0.0, L),) * 3, mass=1.0, gw=gw,
                      filt=False, bins=16, method=ih.IH_RK4, tf=1.0,
                      max_steps=steps, dt=dt, skip=10**9)
         sim.init_gaussian(seed=seeds[r], amplitude=amps[r], k_cut=n * 0.1 / 4)
@@ -222,3 +224,79 @@ def test_autotuned_ensemble_rhs_matches_oracle(ih, E, oracle_mod):
     L.ihg_free(e.ctx, fd)
     L.ihg_free(e.ctx, rd)
     e.close()
+
+
+@pytest.mark.parametrize("shape,nreal,gw,fp32,flags", [
+    ((16, 16, 16), 3, True, False, 1), ((32, 32, 32), 2, False, False, 0),
+    ((128, 128, 128), 1, True, False, 0), ((16, 16, 16), 2, True, True, 1),
+    ((16, 256, 16), 2, True, False, 0)])
+def test_mk_rhs_rk4_equals_rhs_then_stage(ih, shape, nreal, gw, fp32, flags):
+    """ihg_mk_rhs_rk4 (the stage or the final RK4 sum written by the kernels
+    that write the slope) equals ihg_mk_rhs_ensemble followed by
+    ihg_rk4_stage / ihg_rk4_final, bit for bit: 16^3 and 128^3 with GW
+    (fused TT pass), 32^3 without GW, and the FP32 GW source and 256-point
+    y lines (no fused TT pass: the separate calls run)."""
+    L = ih.lib()
+    p = ih.make_params(shape, BOX_CONFIG, 4.0, gw, True, 16, fp32)
+    h = ctypes.c_void_p()
+    ih._check(L.ihg_create_ensemble(ctypes.byref(p), nreal, ctypes.byref(h)))
+    ctx = h.value
+    ens = (ctypes.c_long * 2)()
+    ih._check(L.ihg_ensemble(ctx, ens))
+    n = nreal * ens[1]
+    rng = np.random.default_rng(5)
+
+    def dev(x=None):
+        q = ctypes.c_void_p()
+        ih._check(L.ihg_malloc(ctx, ctypes.byref(q), n))
+        if x is not None:
+            ih._check(L.ihg_upload(ctx, q, x.ctypes.data_as(ctypes.c_void_p), n))
+        return q
+
+    def host(q):
+        out = np.empty(n)
+        ih._check(L.ihg_download(ctx, out.ctypes.data_as(ctypes.c_void_p), q, n))
+        return out
+
+    ys = np.zeros((nreal, ens[1]))
+    for r in range(nreal):
+        y0 = make_state(shape, seed=60 + r, gw=gw, a=1.2 + 0.1 * r)
+        ys[r, :y0.size] = y0
+    ys = ys.ravel()
+    # the input of the stage RHS: y plus a perturbation (as a stage input is)
+    fin = ys * (1.0 + 1e-3 * rng.standard_normal(n))
+    # slopes on the state's elements as an RHS leaves them: zero h blocks
+    # with GW off, zero padding between realizations
+    N = int(np.prod(shape))
+    ntot = sizes(shape)[3]
+    live = np.zeros((nreal, ens[1]), dtype=bool)
+    live[:, :4 * N] = True
+    if gw:
+        live[:, 4 * N:ntot - 1] = True
+    live[:, ntot - 1] = True
+    live = live.ravel()
+    ks = [np.where(live, rng.standard_normal(n) * 1e-2, 0.0) for _ in range(3)]
+    dt = 0.01
+    for mode, half in ((ih.IHG_RK4_STAGE, 1), (ih.IHG_RK4_STAGE, 0),
+                       (ih.IHG_RK4_FINAL, 0)):
+        print(f"ihg_mk_rhs_rk4 {shape} x {nreal}, mode {mode}, half {half}", flush=True)
+        # fused
+        y1, f1, r1, o1 = dev(ys), dev(fin), dev(), dev()
+        k1, k2, k3 = dev(ks[0]), dev(ks[1]), dev(ks[2])
+        ep = ih.IhgRk4Ep(mode, half, dt, y1.value, o1.value, k1.value, k2.value,
+                         k3.value)
+        ih._check(L.ihg_mk_rhs_rk4(ctx, 0.0, f1, r1, flags, ctypes.byref(ep)))
+        # separate
+        y2, f2, r2, o2 = dev(ys), dev(fin), dev(), dev()
+        ih._check(L.ihg_mk_rhs_ensemble(ctx, 0.0, f2, r2, flags))
+        if mode == ih.IHG_RK4_STAGE:
+            ih._check(L.ihg_rk4_stage(ctx, o2, y2, r2, dt, half))
+            assert np.array_equal(host(o1)[live], host(o2)[live])
+            assert np.array_equal(host(r1)[live], host(r2)[live])
+        else:
+            ih._check(L.ihg_rk4_final(ctx, y2, k1, k2, k3, r2, dt))
+            assert np.array_equal(host(y1)[live], host(y2)[live])
+        assert np.array_equal(host(f1), host(f2))  # the filtered input
+        for q in (y1, f1, r1, o1, k1, k2, k3, y2, f2, r2, o2):
+            L.ihg_free(ctx, q)
+    L.ihg_destroy(ctx)
