@@ -217,12 +217,32 @@ struct Lds {
     __device__ static __forceinline__ int idx(int n) { return n + (n >> 4); }
 };
 
+// Synchronisation of the threads of a line between two Stockham stages.
+// WAVE = false: the line's threads may sit in several wavefronts (strided
+// passes put lanes on B neighbouring lines), so the work-group barrier.
+// WAVE = true: the line's threads are lanes of one wavefront (z passes,
+// where a line's L / E threads are consecutive and L / E divides 64), and
+// the wavefront's LDS accesses are performed in order: only the compiler
+// must keep them in order, and the other wavefronts of the work-group run
+// on without waiting (the transposes are wavefront-local exchanges).
+template <bool WAVE>
+__device__ __forceinline__ void line_sync()
+{
+    if constexpr (WAVE) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
 // One Stockham stage of radix R on a line in LDS.  Ns = product of the
 // radices of the earlier stages.  When FIRST, inputs come from rd(slot, n)
 // (point n; slot = m * R + r is the register the value lands in) instead of
 // LDS; when LAST_TO_WR, outputs go to wr(slot, n, value) instead of LDS.
 template <int L, int R, int Ns, int DIR, bool FIRST, bool LAST_TO_WR,
-          class P = Plan<L>, class C, class Rd, class Wr>
+          class P = Plan<L>, class C, bool WAVE = false, class Rd, class Wr>
 __device__ __forceinline__ void stockham_stage(C *line, int j,
                                                const C *__restrict__ tw,
                                                Rd &rd, Wr &wr)
@@ -245,7 +265,7 @@ __device__ __forceinline__ void stockham_stage(C *line, int j,
         }
     }
     if constexpr (!FIRST && !LAST_TO_WR)
-        __syncthreads();
+        line_sync<WAVE>();
 #pragma unroll
     for (int m = 0; m < NB; ++m) {
         const int jb = j + m * T;
@@ -272,7 +292,7 @@ __device__ __forceinline__ void stockham_stage(C *line, int j,
         }
     }
     if constexpr (!LAST_TO_WR)
-        __syncthreads();
+        line_sync<WAVE>();
 }
 
 // Point index of register slot `slot` (= m * R0 + r) in the first stage of
@@ -323,6 +343,34 @@ struct NoWrite {
 // transformed line is in LDS in natural order and the block is synchronised,
 // unless LAST_TO_WR: then `wr` receives every output point and the LDS line
 // may still be read by other threads of the block (no trailing barrier).
+// WAVE: the line's threads are lanes of one wavefront (line_sync); on exit
+// only this wavefront is synchronised.
+template <int L, int DIR, bool FIRST_FROM_RD, bool LAST_TO_WR, class P,
+          class C, bool WAVE, class Rd, class Wr>
+__device__ __forceinline__ void fft_line_impl(C *line, int j,
+                                              const C *__restrict__ tw, Rd rd,
+                                              Wr wr)
+{
+    static_assert(P::E % P::R0 == 0 && P::E % P::R1 == 0 &&
+                      P::E % P::R2 == 0 && P::E % P::R3 == 0,
+                  "every radix divides E");
+    static_assert(P::R0 * P::R1 * P::R2 * P::R3 == L, "radices multiply to L");
+    static_assert(!WAVE || 64 % (L / P::E) == 0,
+                  "a line's threads within one wavefront");
+    constexpr int NST = 1 + (P::R1 > 1) + (P::R2 > 1) + (P::R3 > 1);
+    stockham_stage<L, P::R0, 1, DIR, FIRST_FROM_RD, LAST_TO_WR && NST == 1, P,
+                   C, WAVE>(line, j, tw, rd, wr);
+    if constexpr (P::R1 > 1)
+        stockham_stage<L, P::R1, P::R0, DIR, false, LAST_TO_WR && NST == 2, P,
+                       C, WAVE>(line, j, tw, rd, wr);
+    if constexpr (P::R2 > 1)
+        stockham_stage<L, P::R2, P::R0 * P::R1, DIR, false,
+                       LAST_TO_WR && NST == 3, P, C, WAVE>(line, j, tw, rd, wr);
+    if constexpr (P::R3 > 1)
+        stockham_stage<L, P::R3, P::R0 * P::R1 * P::R2, DIR, false, LAST_TO_WR,
+                       P, C, WAVE>(line, j, tw, rd, wr);
+}
+
 template <int L, int DIR, bool FIRST_FROM_RD = false, bool LAST_TO_WR = false,
           class P = Plan<L>, class C = double2, class Rd = NoRead,
           class Wr = NoWrite>
@@ -330,22 +378,17 @@ __device__ __forceinline__ void fft_line(C *line, int j,
                                          const C *__restrict__ tw,
                                          Rd rd = Rd(), Wr wr = Wr())
 {
-    static_assert(P::E % P::R0 == 0 && P::E % P::R1 == 0 &&
-                      P::E % P::R2 == 0 && P::E % P::R3 == 0,
-                  "every radix divides E");
-    static_assert(P::R0 * P::R1 * P::R2 * P::R3 == L, "radices multiply to L");
-    constexpr int NST = 1 + (P::R1 > 1) + (P::R2 > 1) + (P::R3 > 1);
-    stockham_stage<L, P::R0, 1, DIR, FIRST_FROM_RD, LAST_TO_WR && NST == 1, P,
-                   C>(line, j, tw, rd, wr);
-    if constexpr (P::R1 > 1)
-        stockham_stage<L, P::R1, P::R0, DIR, false, LAST_TO_WR && NST == 2, P,
-                       C>(line, j, tw, rd, wr);
-    if constexpr (P::R2 > 1)
-        stockham_stage<L, P::R2, P::R0 * P::R1, DIR, false,
-                       LAST_TO_WR && NST == 3, P, C>(line, j, tw, rd, wr);
-    if constexpr (P::R3 > 1)
-        stockham_stage<L, P::R3, P::R0 * P::R1 * P::R2, DIR, false, LAST_TO_WR,
-                       P, C>(line, j, tw, rd, wr);
+    fft_line_impl<L, DIR, FIRST_FROM_RD, LAST_TO_WR, P, C, false>(line, j, tw,
+                                                                  rd, wr);
+}
+
+// A line whose threads are lanes of one wavefront, first stage from `rd`,
+// the result in LDS (the z passes): wavefront-local transposes.
+template <int L, int DIR, class P = Plan<L>, class C = double2, class Rd>
+__device__ __forceinline__ void fft_line_wave(C *line, int j,
+                                              const C *__restrict__ tw, Rd rd)
+{
+    fft_line_impl<L, DIR, true, false, P, C, true>(line, j, tw, rd, NoWrite());
 }
 
 } // namespace ihg

@@ -24,6 +24,9 @@ struct Tile {
     static constexpr int T = L / E;
     static constexpr int B = FFT_WG / T;
     static constexpr int PITCH = Lds<L>::PITCH;
+    // z passes: lines per wavefront (a line's T threads are consecutive
+    // lanes), so each wavefront owns LPW lines and finishes them alone
+    static constexpr int LPW = 64 / T;
 };
 
 // Geometry of a set of strided lines: line l starts at
@@ -323,13 +326,16 @@ __device__ __forceinline__ void z_r2c_body(
         const double u = ldn(ra + n), v = ldn(ra + L + n); // in bounds
         return act ? make_double2(u, v) : make_double2(0.0, 0.0);
     };
-    fft_line<L, -1, true>(lds + bc * TL::PITCH, tid % TL::T, tw, rd);
-    const long row0 = 2 * (long)bk.x * TL::B;
-    for (int idx = tid; idx < 2 * TL::B * MZ; idx += FFT_WG) {
+    fft_line_wave<L, -1>(lds + bc * TL::PITCH, tid % TL::T, tw, rd);
+    // each wavefront splits and stores the 2 LPW rows of its own lines
+    const int w = tid / 64, lane = tid % 64;
+    const long row0 = 2 * ((long)bk.x * TL::B + w * TL::LPW);
+    const double2 *wl = lds + w * TL::LPW * TL::PITCH;
+    for (int idx = lane; idx < 2 * TL::LPW * MZ; idx += 64) {
         const int r = idx / MZ, k = idx - r * MZ;
         if (row0 + r < nrows)
             stn(out + row0 * MZ + idx,
-                herm_split<L>(lds + (r >> 1) * TL::PITCH, k, r & 1));
+                herm_split<L>(wl + (r >> 1) * TL::PITCH, k, r & 1));
     }
 }
 
@@ -363,12 +369,15 @@ __device__ __forceinline__ void z_c2r_body(
         const double2 z = herm_merge<L>(A, A + MZ, n); // in bounds for every lane
         return act ? z : make_double2(0.0, 0.0);
     };
-    fft_line<L, +1, true>(lds + bc * TL::PITCH, tid % TL::T, tw, rd);
-    const long row0 = 2 * (long)bk.x * TL::B;
-    for (int idx = tid; idx < 2 * TL::B * L; idx += FFT_WG) {
+    fft_line_wave<L, +1>(lds + bc * TL::PITCH, tid % TL::T, tw, rd);
+    // each wavefront stores the 2 LPW rows of its own lines
+    const int w = tid / 64, lane = tid % 64;
+    const long row0 = 2 * ((long)bk.x * TL::B + w * TL::LPW);
+    const double2 *wl = lds + w * TL::LPW * TL::PITCH;
+    for (int idx = lane; idx < 2 * TL::LPW * L; idx += 64) {
         const int r = idx / L, n = idx - r * L;
         if (row0 + r < nrows) {
-            const double2 z = lds[(r >> 1) * TL::PITCH + Lds<L>::idx(n)];
+            const double2 z = wl[(r >> 1) * TL::PITCH + Lds<L>::idx(n)];
             stn(out + row0 * L + idx, (r & 1) ? z.y : z.x);
         }
     }

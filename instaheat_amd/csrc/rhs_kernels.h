@@ -106,16 +106,21 @@ __device__ __forceinline__ void z_c2r_rho_body(
         const double2 z = herm_merge<L>(A, Bq, n); // in bounds for every lane
         return act ? z : make_double2(0.0, 0.0);
     };
-    // phi, dphi, psi of the points this thread finishes (K = E / 2 per
-    // thread) are loaded before the transform, so their latency hides
-    // behind it; rows past the end read row 0 (in bounds, unused)
-    constexpr int K = RPB * L / FFT_WG;
-    static_assert(K * FFT_WG == RPB * L, "whole rows per thread sweep");
-    const long row0 = (long)bk.x * RPB;
+    // Each wavefront finishes the RPW = LPW / 2 rows of its own lines (line
+    // 2 rr holds (d_x, d_y) of row rr, line 2 rr + 1 (d_z, lap)), K = E / 2
+    // points per lane.  phi, dphi, psi of those points are loaded before
+    // the transform, so their latency hides behind it; rows past the end
+    // read row 0 (in bounds, unused)
+    constexpr int RPW = TL::LPW / 2;
+    constexpr int K = RPW * L / 64;
+    static_assert(K * 64 == RPW * L, "whole rows per wavefront sweep");
+    const int w = tid / 64, lane = tid % 64;
+    const long row0 = (long)bk.x * RPB + w * RPW;
+    const double2 *wl = lds + w * TL::LPW * TL::PITCH;
     double php[K], dphp[K], psp[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        const int idx = tid + k * FFT_WG;
+        const int idx = lane + k * 64;
         const int rr = idx / L, n = idx - rr * L;
         const long r = row0 + rr;
         const long i = (r < nrows ? r : 0) * L + n;
@@ -123,7 +128,7 @@ __device__ __forceinline__ void z_c2r_rho_body(
         dphp[k] = f[N + i];
         psp[k] = f[2 * N + i];
     }
-    fft_line<L, +1, true>(lds + bc * TL::PITCH, tid % TL::T, tw, rd);
+    fft_line_wave<L, +1>(lds + bc * TL::PITCH, tid % TL::T, tw, rd);
 
     const long Ntot = 4 * N + 4 * (2 * M) + 1;
     const double aa = f[Ntot - 1];
@@ -131,12 +136,12 @@ __device__ __forceinline__ void z_c2r_rho_body(
     double srho = 0.0, sp = 0.0;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        const int idx = tid + k * FFT_WG;
+        const int idx = lane + k * 64;
         const int rr = idx / L, n = idx - rr * L;
         const long r = row0 + rr;
         if (r >= nrows) continue;
-        const double2 z1 = lds[(2 * rr) * TL::PITCH + Lds<L>::idx(n)];
-        const double2 z2 = lds[(2 * rr + 1) * TL::PITCH + Lds<L>::idx(n)];
+        const double2 z1 = wl[(2 * rr) * TL::PITCH + Lds<L>::idx(n)];
+        const double2 z2 = wl[(2 * rr + 1) * TL::PITCH + Lds<L>::idx(n)];
         const double dx = z1.x, dy = z1.y, dz = z2.x, lap = z2.y;
         double g = dx * dx;
         g += dy * dy;
@@ -308,17 +313,19 @@ __device__ __forceinline__ void sources_pair(C *lds, int tid, const Rows &g,
         }
         return act ? CT<C>::make(u, v) : CT<C>::make(0, 0);
     };
-    fft_line<L, -1, true>(lds + bc * TL::PITCH, tid % TL::T, tw, rd);
-    for (int idx = tid; idx < TL::B * 2 * MZ; idx += FFT_WG) {
-        const int lb = idx / (2 * MZ);
-        const int rem = idx - lb * 2 * MZ;
+    fft_line_wave<L, -1>(lds + bc * TL::PITCH, tid % TL::T, tw, rd);
+    // each wavefront splits and stores the rows of its own lines
+    const int w = tid / 64, lane = tid % 64;
+    for (int idx = lane; idx < TL::LPW * 2 * MZ; idx += 64) {
+        const int lb = w * TL::LPW + idx / (2 * MZ);
+        const int rem = idx - (lb - w * TL::LPW) * 2 * MZ;
         const int s = rem / MZ, k = rem - s * MZ;
         const long r = row0 + lb;
         if (r < nrows)
             W[(2 * Q + s) * M + r * MZ + k] =
                 herm_split<L>(lds + lb * TL::PITCH, k, s);
     }
-    __syncthreads(); // split reads done before the next transform
+    line_sync<true>(); // split reads done before the next transform
 }
 
 template <int L, class R>
