@@ -241,9 +241,11 @@ __device__ __forceinline__ void line_sync()
 // radices of the earlier stages.  When FIRST, inputs come from rd(slot, n)
 // (point n; slot = m * R + r is the register the value lands in) instead of
 // LDS; when LAST_TO_WR, outputs go to wr(slot, n, value) instead of LDS.
+// The stage reads `line` and writes `out`; when they are different buffers
+// (ping-pong) no barrier is needed between the reads and the writes.
 template <int L, int R, int Ns, int DIR, bool FIRST, bool LAST_TO_WR,
           class P = Plan<L>, class C, bool WAVE = false, class Rd, class Wr>
-__device__ __forceinline__ void stockham_stage(C *line, int j,
+__device__ __forceinline__ void stockham_stage(C *line, C *out, int j,
                                                const C *__restrict__ tw,
                                                Rd &rd, Wr &wr)
 {
@@ -265,7 +267,8 @@ __device__ __forceinline__ void stockham_stage(C *line, int j,
         }
     }
     if constexpr (!FIRST && !LAST_TO_WR)
-        line_sync<WAVE>();
+        if (out == line)
+            line_sync<WAVE>();
 #pragma unroll
     for (int m = 0; m < NB; ++m) {
         const int jb = j + m * T;
@@ -288,7 +291,7 @@ __device__ __forceinline__ void stockham_stage(C *line, int j,
             if constexpr (LAST_TO_WR)
                 wr(m * R + r, base + r * Ns, v[m][r]);
             else
-                line[Lds<L>::idx(base + r * Ns)] = v[m][r];
+                out[Lds<L>::idx(base + r * Ns)] = v[m][r];
         }
     }
     if constexpr (!LAST_TO_WR)
@@ -347,7 +350,7 @@ struct NoWrite {
 // only this wavefront is synchronised.
 template <int L, int DIR, bool FIRST_FROM_RD, bool LAST_TO_WR, class P,
           class C, bool WAVE, class Rd, class Wr>
-__device__ __forceinline__ void fft_line_impl(C *line, int j,
+__device__ __forceinline__ void fft_line_impl(C *line, C *line2, int j,
                                               const C *__restrict__ tw, Rd rd,
                                               Wr wr)
 {
@@ -358,17 +361,27 @@ __device__ __forceinline__ void fft_line_impl(C *line, int j,
     static_assert(!WAVE || 64 % (L / P::E) == 0,
                   "a line's threads within one wavefront");
     constexpr int NST = 1 + (P::R1 > 1) + (P::R2 > 1) + (P::R3 > 1);
+    // buffers of stages 2.. (line2 == line: in place)
+    C *const b0 = line, *const b1 = line2;
     stockham_stage<L, P::R0, 1, DIR, FIRST_FROM_RD, LAST_TO_WR && NST == 1, P,
-                   C, WAVE>(line, j, tw, rd, wr);
+                   C, WAVE>(b0, FIRST_FROM_RD ? b0 : b1, j, tw, rd, wr);
+    C *const s2 = FIRST_FROM_RD ? b0 : b1, *const d2 = FIRST_FROM_RD ? b1 : b0;
     if constexpr (P::R1 > 1)
         stockham_stage<L, P::R1, P::R0, DIR, false, LAST_TO_WR && NST == 2, P,
-                       C, WAVE>(line, j, tw, rd, wr);
+                       C, WAVE>(s2, d2, j, tw, rd, wr);
     if constexpr (P::R2 > 1)
         stockham_stage<L, P::R2, P::R0 * P::R1, DIR, false,
-                       LAST_TO_WR && NST == 3, P, C, WAVE>(line, j, tw, rd, wr);
+                       LAST_TO_WR && NST == 3, P, C, WAVE>(d2, s2, j, tw, rd, wr);
     if constexpr (P::R3 > 1)
         stockham_stage<L, P::R3, P::R0 * P::R1 * P::R2, DIR, false, LAST_TO_WR,
-                       P, C, WAVE>(line, j, tw, rd, wr);
+                       P, C, WAVE>(s2, d2, j, tw, rd, wr);
+}
+
+// number of Stockham stages of a plan
+template <class P>
+constexpr int plan_stages()
+{
+    return 1 + (P::R1 > 1) + (P::R2 > 1) + (P::R3 > 1);
 }
 
 template <int L, int DIR, bool FIRST_FROM_RD = false, bool LAST_TO_WR = false,
@@ -378,8 +391,21 @@ __device__ __forceinline__ void fft_line(C *line, int j,
                                          const C *__restrict__ tw,
                                          Rd rd = Rd(), Wr wr = Wr())
 {
-    fft_line_impl<L, DIR, FIRST_FROM_RD, LAST_TO_WR, P, C, false>(line, j, tw,
-                                                                  rd, wr);
+    fft_line_impl<L, DIR, FIRST_FROM_RD, LAST_TO_WR, P, C, false>(line, line, j,
+                                                                  tw, rd, wr);
+}
+
+// A line transform from `rd` to `wr` whose stages alternate between the LDS
+// lines `line` and `line2` (ping-pong): one work-group barrier per stage
+// instead of two.  The last stage reads `line` when the plan has an even
+// number of stages and `line2` otherwise (the first stage writes `line`).
+template <int L, int DIR, class P = Plan<L>, class C = double2, class Rd,
+          class Wr>
+__device__ __forceinline__ void fft_line_pp(C *line, C *line2, int j,
+                                            const C *__restrict__ tw, Rd rd,
+                                            Wr wr)
+{
+    fft_line_impl<L, DIR, true, true, P, C, false>(line, line2, j, tw, rd, wr);
 }
 
 // A line whose threads are lanes of one wavefront, first stage from `rd`,
@@ -388,7 +414,8 @@ template <int L, int DIR, class P = Plan<L>, class C = double2, class Rd>
 __device__ __forceinline__ void fft_line_wave(C *line, int j,
                                               const C *__restrict__ tw, Rd rd)
 {
-    fft_line_impl<L, DIR, true, false, P, C, true>(line, j, tw, rd, NoWrite());
+    fft_line_impl<L, DIR, true, false, P, C, true>(line, line, j, tw, rd,
+                                                   NoWrite());
 }
 
 } // namespace ihg

@@ -347,7 +347,7 @@ struct FamTT {
     template <int L>
     static constexpr int lds()
     {
-        return (WG / (L / GroupTTPlan<L>::P::E)) * Lds<L>::PITCH;
+        return 2 * (WG / (L / GroupTTPlan<L>::P::E)) * Lds<L>::PITCH;
     }
     template <int L>
     __device__ static void run(const Blk &bk, double2 *lds, const Args &a)

@@ -423,17 +423,22 @@ __device__ __forceinline__ void tt_coef(R kx, R ky, R kz, R inv_ksq,
 
 // y transform of source component C (its z and x passes are done) with the
 // TT accumulation folded into the last Stockham stage; `cur` holds the
-// component's first-stage points (loaded one component ahead)
+// component's first-stage points (loaded one component ahead).  The stages
+// alternate between the LDS lines my[0] and my[1] (ping-pong: one barrier
+// per stage), and each component starts on the line its predecessor's last
+// stage did not read, so no barrier is needed between components.
 template <int L, class P, int K, class C>
 __device__ __forceinline__ void tt_component(
-    C *my, int j, const C *__restrict__ tw, const C *cur, const KGeom &kg,
-    int ix, int iz, const typename CT<C>::R *inv_ksq, C *o0, C *o1)
+    C *const *my, int j, const C *__restrict__ tw, const C *cur,
+    const KGeom &kg, int ix, int iz, const typename CT<C>::R *inv_ksq, C *o0,
+    C *o1)
 {
     using R = typename CT<C>::R;
     const R kx = (R)kg.kxf(ix), kz = (R)kg.kzf(iz);
     const bool b1 = fabs(kg.kzf(iz)) > DBL_EPSILON; // branch on the FP64 k
-    if constexpr (K > 0)
-        __syncthreads(); // LDS reads of the previous component are done
+    // an even number of stages ends reading the first line: swap per
+    // component; an odd number ends reading the second: keep the order
+    constexpr int PAR = plan_stages<P>() % 2 == 0 ? K % 2 : 0;
     // Register pressure control (empty asm statements, no instructions):
     // an opaque copy of the twiddle pointer per component stops the compiler
     // from loading every stage twiddle once and holding it across all six
@@ -459,7 +464,7 @@ __device__ __forceinline__ void tt_component(
             o1[s] = CT<C>::make(o1[s].x + be * v.x, o1[s].y + be * v.y);
         }
     };
-    fft_line<L, -1, true, true, P>(my, j, twc, rd, acc);
+    fft_line_pp<L, -1, P>(my[PAR], my[1 - PAR], j, twc, rd, acc);
 #pragma unroll
     for (int s = 0; s < P::E; ++s)
         asm volatile("" : "+v"(o0[s].x), "+v"(o0[s].y), "+v"(o1[s].x),
@@ -504,7 +509,8 @@ __device__ __forceinline__ void y_tt_update_h_body(
     const long base = ls.base(lc);
     const int ix = (int)(lc / kg.Mz), iz = (int)(lc - (long)ix * kg.Mz);
     const long st = ls.stride;
-    C *my = lds + b * PITCH;
+    // the two ping-pong lines of this thread's line (lds holds 2 B lines)
+    C *const my[2] = {lds + b * PITCH, lds + (B + b) * PITCH};
     R inv[E];
 #pragma unroll
     for (int s = 0; s < E; ++s) {
@@ -601,7 +607,7 @@ __global__ __launch_bounds__(WG) void k_y_tt_update_h(
     const double *__restrict__ dscal, LineSet ls, const C *__restrict__ tw,
     Ens e, long rw)
 {
-    __shared__ C lds[(WG / (L / P::E)) * Lds<L>::PITCH];
+    __shared__ C lds[2 * (WG / (L / P::E)) * Lds<L>::PITCH];
     y_tt_update_h_body<L, WG, P, C>(hw_blk(), lds, W, M, f, result, N, Next,
                                     Ntot, kg, dscal, ls, tw, e, rw);
 }
