@@ -18,14 +18,25 @@ namespace ihg {
 
 constexpr int FFT_WG = 256;
 
+// Line plans of the z passes.  Their transposes are wavefront-local, so an
+// extra stage costs LDS traffic but no work-group barrier, and fewer values
+// per thread buy occupancy: 256 points take 8 values per thread in three
+// stages (at 16 values the z c2r + rho pass held 182 registers and 70 KB of
+// LDS per work-group, 2 waves per SIMD).  (512 points keep 16 values: a
+// wavefront must hold both lines of a row pair.)
+template <int L> struct ZPlanOf { using P = Plan<L>; };
+template <> struct ZPlanOf<256> { using P = PlanT<8, 8, 8, 4>; };
+
+// Tile of a z pass: B lines of L points per work-group, T threads per line
 template <int L>
 struct Tile {
-    static constexpr int E = Plan<L>::E;
+    using P = typename ZPlanOf<L>::P;
+    static constexpr int E = P::E;
     static constexpr int T = L / E;
     static constexpr int B = FFT_WG / T;
     static constexpr int PITCH = Lds<L>::PITCH;
-    // z passes: lines per wavefront (a line's T threads are consecutive
-    // lanes), so each wavefront owns LPW lines and finishes them alone
+    // lines per wavefront (a line's T threads are consecutive lanes), so
+    // each wavefront owns LPW lines and finishes them alone
     static constexpr int LPW = 64 / T;
 };
 
@@ -326,7 +337,7 @@ __device__ __forceinline__ void z_r2c_body(
         const double u = ldn(ra + n), v = ldn(ra + L + n); // in bounds
         return act ? make_double2(u, v) : make_double2(0.0, 0.0);
     };
-    fft_line_wave<L, -1>(lds + bc * TL::PITCH, tid % TL::T, tw, rd);
+    fft_line_wave<L, -1, typename TL::P>(lds + bc * TL::PITCH, tid % TL::T, tw, rd);
     // each wavefront splits and stores the 2 LPW rows of its own lines
     const int w = tid / 64, lane = tid % 64;
     const long row0 = 2 * ((long)bk.x * TL::B + w * TL::LPW);
@@ -369,7 +380,7 @@ __device__ __forceinline__ void z_c2r_body(
         const double2 z = herm_merge<L>(A, A + MZ, n); // in bounds for every lane
         return act ? z : make_double2(0.0, 0.0);
     };
-    fft_line_wave<L, +1>(lds + bc * TL::PITCH, tid % TL::T, tw, rd);
+    fft_line_wave<L, +1, typename TL::P>(lds + bc * TL::PITCH, tid % TL::T, tw, rd);
     // each wavefront stores the 2 LPW rows of its own lines
     const int w = tid / 64, lane = tid % 64;
     const long row0 = 2 * ((long)bk.x * TL::B + w * TL::LPW);

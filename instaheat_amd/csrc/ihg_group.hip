@@ -504,9 +504,17 @@ static Seg<FamStrided<DIR>> strided_seg(const ihg_ctx *c, double2 *data,
     return s;
 }
 
+// lines per work-group of a z pass and of a strided (fan-out) pass
 static int tile_b(int L)
 {
     return dispatch_len(L, [&](auto Lc) { return Tile<decltype(Lc)::value>::B; });
+}
+
+static int strided_b(int L)
+{
+    return dispatch_len(L, [&](auto Lc) {
+        return StridedShape<decltype(Lc)::value>::B;
+    });
 }
 
 static unsigned upd_blocks(long n)
@@ -620,7 +628,7 @@ static int group_rhs(ihg_group *g, double *const *f, double *const *result,
                                 c->Mz, (double)N, c->W + 4 * M,
                                 nr > 1 ? nullptr : c->Wc,
                                 nr > 1 ? nullptr : c->Wc + M, e.w};
-            const int Bx = tile_b(L);
+            const int Bx = strided_b(L);
             Seg<FamFanout<1, true, FilterFac>> sx;
             sx.L = L;
             sx.grid = dim3((unsigned)((ls.nlines + Bx - 1) / Bx), 4, nr);
@@ -661,7 +669,8 @@ static int group_rhs(ihg_group *g, double *const *f, double *const *result,
             }
             const LineSet ls = x_lines_of(c);
             const DerivFac fac{c->kg, (double)N};
-            const dim3 gx((unsigned)((ls.nlines + B - 1) / B), 1, nr);
+            const int Bx = strided_b(L);
+            const dim3 gx((unsigned)((ls.nlines + Bx - 1) / Bx), 1, nr);
             if (kept[i]) {
                 Seg<FamFanout<4, false, DerivFac>> s;
                 s.L = L;

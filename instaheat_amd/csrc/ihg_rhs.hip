@@ -303,7 +303,7 @@ int rhs_filter(ihg_ctx *c, double *f, int nr)
     IHG_TRY(y_pass(c, W, -1, 4, M, nr, e.w));
     IHG_TRY(dispatch_len(c->Nx, [&](auto Lc) {
         constexpr int L = decltype(Lc)::value;
-        using TL = Tile<L>;
+        using SS = StridedShape<L>;
         const LineSet ls = x_lineset(c);
         // one realization: keep phi_hat in W4 and Wc, psi_hat in Wc + M;
         // an ensemble (no output calls) keeps phi_hat in each W4
@@ -313,7 +313,7 @@ int rhs_filter(ihg_ctx *c, double *f, int nr)
         // four fields in and out, three kept spectra out
         return timed(c, F_X_ROUNDTRIP, 11.0 * 16.0 * M * nr, [&]() {
             hipLaunchKernelGGL((k_x_fanout<L, 1, true, FilterFac>),
-                               dim3((unsigned)((ls.nlines + TL::B - 1) / TL::B), 4, nr),
+                               dim3((unsigned)((ls.nlines + SS::B - 1) / SS::B), 4, nr),
                                dim3(FFT_WG), 0, c->stream, W, W, 0L, ls, M,
                                c->tw, fac, e.w);
             return hip_check(hipGetLastError(), "k_x_fanout (filter)");

@@ -112,6 +112,7 @@ __device__ __forceinline__ void z_c2r_rho_body(
     // the transform, so their latency hides behind it; rows past the end
     // read row 0 (in bounds, unused)
     constexpr int RPW = TL::LPW / 2;
+    static_assert(RPW >= 1, "both lines of a row in one wavefront");
     constexpr int K = RPW * L / 64;
     static_assert(K * 64 == RPW * L, "whole rows per wavefront sweep");
     const int w = tid / 64, lane = tid % 64;
@@ -128,7 +129,7 @@ __device__ __forceinline__ void z_c2r_rho_body(
         dphp[k] = f[N + i];
         psp[k] = f[2 * N + i];
     }
-    fft_line_wave<L, +1>(lds + bc * TL::PITCH, tid % TL::T, tw, rd);
+    fft_line_wave<L, +1, typename TL::P>(lds + bc * TL::PITCH, tid % TL::T, tw, rd);
 
     const long Ntot = 4 * N + 4 * (2 * M) + 1;
     const double aa = f[Ntot - 1];
@@ -203,10 +204,14 @@ __device__ __forceinline__ void means_body(
     f += r * e.f;
     result += r * e.f;
     dscal += r * e.s;
+    // (rho, p) pairs as 16-byte loads, four in flight per thread
+    const double2 *p2 = reinterpret_cast<const double2 *>(partials);
     double s0 = 0.0, s1 = 0.0;
+#pragma unroll 4
     for (int i = threadIdx.x; i < nparts; i += RED_WG) {
-        s0 += partials[2 * i];
-        s1 += partials[2 * i + 1];
+        const double2 v = p2[i];
+        s0 += v.x;
+        s1 += v.y;
     }
     s0 = block_reduce<OpSum>(s0, sh, RED_WG);
     s1 = block_reduce<OpSum>(s1, sh, RED_WG);
@@ -313,7 +318,7 @@ __device__ __forceinline__ void sources_pair(C *lds, int tid, const Rows &g,
         }
         return act ? CT<C>::make(u, v) : CT<C>::make(0, 0);
     };
-    fft_line_wave<L, -1>(lds + bc * TL::PITCH, tid % TL::T, tw, rd);
+    fft_line_wave<L, -1, typename TL::P>(lds + bc * TL::PITCH, tid % TL::T, tw, rd);
     // each wavefront splits and stores the rows of its own lines
     const int w = tid / 64, lane = tid % 64;
     for (int idx = lane; idx < TL::LPW * 2 * MZ; idx += 64) {
@@ -330,7 +335,7 @@ __device__ __forceinline__ void sources_pair(C *lds, int tid, const Rows &g,
 
 template <int L, class R>
 struct GradRegs {
-    R x[Plan<L>::E], y[Plan<L>::E], z[Plan<L>::E];
+    R x[Tile<L>::E], y[Tile<L>::E], z[Tile<L>::E];
 };
 
 // C = double2: the FP64 path; C = float2: the precision form (gradients
@@ -354,7 +359,7 @@ __device__ __forceinline__ void z_r2c_sources_body(
     GradRegs<L, R> g;
 #pragma unroll
     for (int s = 0; s < TL::E; ++s) {
-        const int n = stage1_point<L>(j, s);
+        const int n = stage1_point<L, typename TL::P>(j, s);
         g.x[s] = gx[n];
         g.y[s] = gx[N + n];
         g.z[s] = gx[2 * N + n];
