@@ -1,4 +1,5 @@
-"""GPU parity at the full workloads of the three added forms:
+"""GPU parity at the full workloads of the three added forms and of the
+64^3 DOPRI853 configuration:
 
 * precision: 256^3, k_min 0.05, the GW source and spectrum in FP32 -- one
   filtered output RHS against the FP64 oracle, and the GW spectrum per k bin
@@ -11,7 +12,10 @@
   realization's RHS per size against the FP64 oracle;
 * learned: 256^3 FP64 DOPRI853 rtol 1e-10 with the step proposer from a
   state in the error-controlled phase (t > 20) -- its end state after four
-  accepted steps against the oracle taking the same steps.
+  accepted steps against the oracle taking the same steps;
+* the 64^3 DOPRI853 configuration with GW on, outputs every 0.5 and 64
+  log-spaced bins, t = 0 to 100 -- every output saved, and 6 steps from
+  t = 50 against the oracle taking the same steps.
 
 Each case prints its name and shape before it touches the GPU."""
 import numpy as np
@@ -255,4 +259,73 @@ def test_tuned_form_256_matches_oracle_on_its_steps(ih, oracle_mod):
     assert ro["t"] == pytest.approx(t0 + dts.sum(), rel=1e-14)
     err = rel_l2(yg, yo)
     print(f"256^3 tuned form end state vs the oracle on its 4 steps: {err:.3e}")
+    assert err < 1e-9
+
+
+@pytest.mark.timeout(900)
+def test_config2_dopri853_to_t100(ih, oracle_mod):
+    """The 64^3 DOPRI853 configuration at its workload: GW and filter on,
+    k_min 0.1, amplitude 1e-4, rtol 1e-10, atol 1e-14, outputs every 0.5
+    time units (dense output) with 64 log-spaced spectrum bins, t = 0 to
+    100.  At t = 50 the oracle takes the GPU's next 6 accepted steps from
+    the same state, and the end states agree within 1e-9; every one of the
+    200 output times is saved, with finite observables and positive
+    spectra; the scale factor grows at every save."""
+    n, bins, kmin = 64, 64, 0.1
+    bounds = ((0.0, 2 * np.pi / kmin),) * 3
+    kw = dict(bounds=bounds, mass=1.0, gw=True, filt=True, bins=bins)
+    t_out = np.arange(1, 201) * 0.5
+    sim = ih.Sim((n, n, n), method=ih.IH_DOPRI853, ti=0.0, tf=50.0, dt=1e-7,
+                 rtol=1e-10, atol=1e-14, hubble_frac=1e-2, skip=10 ** 9,
+                 log_bins=True, **kw)
+    sim.init_gaussian(seed=2, amplitude=1e-4, k_cut=n * kmin / 4)
+    sim.mk_initial_psi()
+    saves = []
+    print("config 2, 64^3 DOPRI853: t = 0 to 50", flush=True)
+    r1 = sim.run_dopri853(saves=saves, t_out=t_out[t_out <= 50.0])
+    assert sim.t == pytest.approx(50.0, rel=1e-14)
+    # the window the oracle replays: 6 accepted steps from t = 50
+    y50, t50 = sim.field(), sim.t
+    sim.set_tf(100.0)
+    sim.set_dt(r1["dt_next"])
+    sim.set_max_steps(5)
+    rw = sim.run_dopri853(saves=saves, t_out=t_out[t_out > t50], log_cap=100)
+    log = rw["log"]
+    dts = log[log[:, 7] > 0.5, 1]
+    assert len(dts) == 6
+    yw, tw = sim.field(), sim.t
+    print(f"config 2: window of 6 steps from t = {t50} to {tw}; to t = 100",
+          flush=True)
+    sim.set_max_steps(10 ** 15)
+    sim.set_dt(rw["dt_next"])
+    r2 = sim.run_dopri853(saves=saves, t_out=t_out[t_out > tw])
+    assert sim.t == pytest.approx(100.0, rel=1e-14)
+    n_ok = r1["n_ok"] + rw["n_ok"] + r2["n_ok"]
+    n_bad = r1["n_bad"] + rw["n_bad"] + r2["n_bad"]
+    print(f"config 2: {n_ok} + {n_bad} rejected steps to t = 100, "
+          f"{ih.lib().ihg_calls_rhs(sim.ctx)} RHS, a = {sim.field()[-1]:.4f}",
+          flush=True)
+    assert n_bad < 0.1 * n_ok
+    times = np.array([t for t, _, _ in saves])
+    for t in t_out:
+        assert np.any(np.abs(times - t) < 1e-9), t
+    a_prev = 0.0
+    for t, a, obs in sorted(saves, key=lambda s: s[0]):
+        o = ih.unpack_obs(obs, bins)
+        assert np.all(np.isfinite(obs)), t
+        assert o["phi_ps"].sum() > 0 and np.all(o["gw"] >= 0), t
+        if t >= 1.0:  # h grows from zero
+            assert o["gw"].sum() > 0, t
+        assert a >= a_prev, t
+        a_prev = a
+    assert a_prev > 1.5
+    sim.close()
+    print("config 2: the oracle replays the 6 steps from t = 50", flush=True)
+    o = oracle_mod.Oracle(n, n, n, **kw)
+    yo = y50.copy()
+    ro = o.run_dopri853(yo, np.zeros_like(yo), t50, 100.0, dts[0], rtol=1e-10,
+                        atol=1e-14, hubble_frac=1e-2, skip=10 ** 9, forced=dts)
+    assert ro["t"] == pytest.approx(tw, rel=1e-14)
+    err = rel_l2(yw, yo)
+    print(f"64^3 end state vs the oracle on the GPU's 6 steps: {err:.3e}")
     assert err < 1e-9

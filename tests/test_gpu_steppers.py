@@ -135,6 +135,37 @@ def test_dopri853_free_running(ih, oracle_mod):
         assert rel_l2(obg, obo) < 1e-8
 
 
+def test_dopri853_free_running_32_to_t3(ih, oracle_mod):
+    """32^3, GW on, the adaptive method left to itself from t = 0 to 3
+    (about 80 steps, growing from 1e-3 by the controller's scaling): the
+    same accepted and rejected steps with the same sizes as the oracle's
+    run, and end states within 1e-9."""
+    shape = (32, 32, 32)
+    o, ro, yo, rg, yg, so, sg = _dp_case(ih, oracle_mod, shape, True, 3.0,
+                                         skip=10)
+    print("oracle", ro["n_stp"], ro["n_ok"], ro["n_bad"], "gpu", rg["n_stp"],
+          rg["n_ok"], rg["n_bad"])
+    assert rg["n_ok"] >= 50
+    assert (rg["n_stp"], rg["n_ok"], rg["n_bad"]) == (ro["n_stp"], ro["n_ok"],
+                                                       ro["n_bad"])
+    dts_g = np.array([r[1] for r in rg["log"]])
+    assert np.allclose(dts_g, ro["dt"], rtol=1e-8)
+    acc_g = np.array([r[7] for r in rg["log"]]) > 0.5
+    assert np.array_equal(acc_g, ro["accepted"])
+    assert rg["t"] == pytest.approx(ro["t"], rel=1e-14)
+    B, keys = _live(shape, True)
+    mask = np.zeros(o.Ntot, bool)
+    for k in keys:
+        mask[B[k]] = True
+    err = rel_l2(yg[mask], yo[mask])
+    print(f"32^3 free-running DOPRI853 to t = 3: end state {err:.3e}")
+    assert err < 1e-9
+    assert len(sg) == len(so)
+    for (tg, ag, obg), (to, ao, obo) in zip(sg, so):
+        assert tg == pytest.approx(to, rel=1e-13)
+        assert rel_l2(obg, obo) < 1e-8
+
+
 def test_stage_kernels_match_numpy(ih):
     """ihg_dp853_stage / _finish and the RK4 kernels against numpy on random
     arrays with the tableau the library reports."""
