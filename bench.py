@@ -68,6 +68,10 @@ def parse():
     ap.add_argument("--copy-probe", type=float, default=0.0, metavar="GB",
                     help="only time a plain copy kernel of GB gigabytes each "
                          "way (the HBM ceiling and the counter calibration)")
+    ap.add_argument("--mlp", action="store_true",
+                    help="only time the step proposer's FP32 kernels: "
+                         "training (Adam epochs over a training-set-sized "
+                         "sample), the batched gradient and inference")
     ap.add_argument("--next-rows", action="store_true",
                     help="only measure the follow-on paths at the workload's "
                          "grid: device initial data, dense output, HDF5 "
@@ -327,6 +331,54 @@ def ensemble_bench(steps):
                                    "mean": v["gw_mean"], "var": v["gw_var"]}
                           for n, v in gg["sizes"].items()}
     return res
+
+
+def mlp_bench(n=419611, epochs=3, batch=1024):
+    """The learned proposer's kernels (ihg_mlp_*): Adam training epochs over
+    n synthetic samples of the training set's size and shape, the gradient
+    of the whole set, and inference of one sample (the per-step proposal)
+    and of the whole set; FLOPs counted as 2 per multiply-add of the three
+    layers (forward) and 3x that for a training sample (forward and
+    backward)."""
+    from instaheat_amd.proposer import Mlp
+    rng = np.random.default_rng(1)
+    X = rng.standard_normal((n, 20)).astype(np.float32)
+    T = (0.1 * X[:, 0] - 0.05 * X[:, 3]).astype(np.float32)
+    m = Mlp(seed=1)
+    macs = 20 * 64 + 64 * 64 + 64
+    m.train(X[:4096], T[:4096], 1, batch=batch)  # warm-up
+    t0 = time.perf_counter()
+    hist = m.train(X, T, epochs, batch=batch)
+    train_s = time.perf_counter() - t0
+    m.grad(X[:4096], T[:4096])
+    t0 = time.perf_counter()
+    for _ in range(3):
+        m.grad(X, T)
+    grad_s = (time.perf_counter() - t0) / 3
+    one = X[:1]
+    for _ in range(10):
+        m.forward(one)
+    t0 = time.perf_counter()
+    for _ in range(200):
+        m.forward(one)
+    fwd1_us = (time.perf_counter() - t0) / 200 * 1e6
+    t0 = time.perf_counter()
+    for _ in range(3):
+        m.forward(X)
+    fwd_s = (time.perf_counter() - t0) / 3
+    m.close()
+    sps = n * epochs / train_s
+    return {"metric": "step-proposer kernels (20-64-64-1 tanh MLP, FP32)",
+            "samples": n, "epochs": epochs, "batch": batch,
+            "train_samples_per_s": round(sps, 1),
+            "train_s_per_epoch": round(train_s / epochs, 4),
+            "train_GFLOP_per_s": round(sps * 6 * macs / 1e9, 2),
+            "grad_full_set_ms": round(grad_s * 1e3, 3),
+            "forward_one_sample_us": round(fwd1_us, 2),
+            "forward_full_set_ms": round(fwd_s * 1e3, 3),
+            "epoch_loss": [float(v) for v in hist],
+            "note": "includes the host-device copies of each call (host "
+                    "arrays at the C ABI)"}
 
 
 def measured_traffic(kernel, grid):
@@ -641,6 +693,9 @@ def main():
         return
     if args.ensemble:
         print(json.dumps(ensemble_bench(args.steps)))
+        return
+    if args.mlp:
+        print(json.dumps(mlp_bench()))
         return
     wl = workload(args.grid)
     if args.next_rows:
