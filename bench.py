@@ -677,7 +677,10 @@ def cpu_baseline(args, wl, check, dt_mean):
                        f"{check['dts'][0]:.4g} from the {n}^3 warm-up state at "
                        f"t = {check['t']:.3f} ({secs:.1f} s, incl. the initial "
                        f"output call), scaled by the GPU plain form's mean dt "
-                       f"{dt_mean:.4g}"),
+                       f"{dt_mean:.4g}; this is the project's own restatement "
+                       f"(radix-2 FFTs), not the reference's FFTW build, so "
+                       f"the GPU/CPU ratio overstates the gain over the "
+                       f"reference"),
             "seconds_per_step": round(step_s, 3),
             "gpu_end_state_rel_err_vs_oracle": err}
 
