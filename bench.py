@@ -68,6 +68,9 @@ def parse():
     ap.add_argument("--copy-probe", type=float, default=0.0, metavar="GB",
                     help="only time a plain copy kernel of GB gigabytes each "
                          "way (the HBM ceiling and the counter calibration)")
+    ap.add_argument("--ensemble-forms", default="eager,graph,tuned,grouped,grouped_graph",
+                    help="with --ensemble: the forms to run, comma separated "
+                         "(eager, graph, tuned, grouped, grouped_graph)")
     ap.add_argument("--mlp", action="store_true",
                     help="only time the step proposer's FP32 kernels: "
                          "training (Adam epochs over a training-set-sized "
@@ -282,25 +285,31 @@ def run_gpu(args, wl):
                 copy_gbs=copy_gbs, pre_steps=pre["n_ok"] + pre["n_bad"])
 
 
-def ensemble_bench(steps):
+ENSEMBLE_FORMS = {"eager": (False, False, False), "graph": (True, False, False),
+                  "tuned": (True, True, False), "grouped": (False, False, True),
+                  "grouped_graph": (True, False, True)}
+
+
+def ensemble_bench(steps, forms=tuple(ENSEMBLE_FORMS)):
     """The batched form: realization-steps per second of the mixed ensemble,
     per size on its own stream (eager launches, one graph per size and step,
     graphs with the autotuned strided passes) and grouped (every pass of all
     sizes in one launch, eager and as one graph per step), with the ensemble
-    mean and variance of the GW spectrum per size (grouped graph run)."""
+    mean and variance of the GW spectrum per size (the last form run).
+    `forms` picks a subset (a profiler run takes the eager ones)."""
     from instaheat_amd import ensemble
+    for f in forms:
+        if f not in ENSEMBLE_FORMS:
+            raise SystemExit(f"unknown ensemble form {f!r}")
     out = {}
-    for name, graph, tune, grouped in (("eager", False, False, False),
-                                       ("graph", True, False, False),
-                                       ("tuned", True, True, False),
-                                       ("grouped", False, False, True),
-                                       ("grouped_graph", True, False, True)):
+    for name in forms:
+        graph, tune, grouped = ENSEMBLE_FORMS[name]
         out[name] = ensemble.run_mixed(nsteps=steps, dt=0.005, use_graph=graph,
                                        autotune=tune, grouped=grouped,
-                                       keep_states=name in ("eager", "grouped_graph"))
+                                       keep_states=True)
         print(f"ensemble {name}: {out[name]['realization_steps_per_s']:.1f} "
               f"realization-steps/s", file=sys.stderr, flush=True)
-    e = out["eager"]
+    base = out.get("eager", out[forms[0]])
 
     def rate(name):
         r = out[name]
@@ -308,28 +317,40 @@ def ensemble_bench(steps):
                 "wall_s": round(r["wall_s"], 3),
                 "ms_per_step": round(1e3 * r["wall_s"] / steps, 3),
                 "over_eager": round(r["realization_steps_per_s"] /
-                                    e["realization_steps_per_s"], 3)}
-    gg = out["grouped_graph"]
+                                    base["realization_steps_per_s"], 3)}
+    head = out.get("grouped_graph", out[forms[-1]])
     res = {"metric": "realization-steps/s of the mixed ensemble "
                      "(48 x 32^3 + 16 x 64^3 + 4 x 128^3, FP64 RK4 dt 0.005, GW on)",
            "steps": steps,
-           "value": round(gg["realization_steps_per_s"], 2),
-           "form": "grouped_graph (one launch per pass for all sizes, one graph per step)",
-           "rhs_evals": gg["rhs_evals"]}
+           "value": round(head["realization_steps_per_s"], 2),
+           "form": ("grouped_graph (one launch per pass for all sizes, one "
+                    "graph per step)") if "grouped_graph" in out else forms[-1],
+           "rhs_evals": head["rhs_evals"]}
     for name in out:
         res[name] = rate(name)
-    res["tuned"]["candidates_yx"] = out["tuned"]["autotuned"]
-    # the grouped steps equal the per-size steps bit for bit; the GW spectra
-    # agree to the rounding of their atomic bin sums
-    res["grouped_states_equal_per_size"] = all(
-        np.array_equal(gg["sizes"][n]["states"], e["sizes"][n]["states"])
-        for n in e["sizes"])
-    res["grouped_spectra_match_per_size"] = all(
-        np.allclose(gg["sizes"][n]["gw_mean"], e["sizes"][n]["gw_mean"],
-                    rtol=1e-12, atol=0) for n in e["sizes"])
+    if "tuned" in out:
+        res["tuned"]["candidates_yx"] = out["tuned"]["autotuned"]
+    # every form's states equal the first form's bit for bit (the tuned
+    # form's strided passes differ in rounding); the GW spectra agree to the
+    # rounding of their atomic bin sums
+    ref = out[forms[0]]
+    res["states_equal"] = {
+        name: all(np.array_equal(r["sizes"][n]["states"], ref["sizes"][n]["states"])
+                  for n in ref["sizes"]) for name, r in out.items() if name != "tuned"}
+    res["spectra_match"] = all(
+        np.allclose(r["sizes"][n]["gw_mean"], ref["sizes"][n]["gw_mean"],
+                    rtol=1e-12, atol=0)
+        for name, r in out.items() if name != "tuned" for n in ref["sizes"])
+    if "tuned" in out:
+        t = out["tuned"]
+        res["tuned"]["gw_mean_rel_diff_max"] = max(
+            float(np.max(np.abs(np.array(t["sizes"][n]["gw_mean"]) -
+                                np.array(ref["sizes"][n]["gw_mean"])) /
+                         np.maximum(np.abs(ref["sizes"][n]["gw_mean"]), 1e-300)))
+            for n in ref["sizes"])
     res["gw_spectrum"] = {str(n): {"nreal": v["nreal"], "t": v["t"],
                                    "mean": v["gw_mean"], "var": v["gw_var"]}
-                          for n, v in gg["sizes"].items()}
+                          for n, v in head["sizes"].items()}
     return res
 
 
@@ -695,7 +716,8 @@ def main():
         print(json.dumps(copy_probe(args.copy_probe)))
         return
     if args.ensemble:
-        print(json.dumps(ensemble_bench(args.steps)))
+        print(json.dumps(ensemble_bench(args.steps,
+                                        args.ensemble_forms.split(","))))
         return
     if args.mlp:
         print(json.dumps(mlp_bench()))
