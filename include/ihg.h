@@ -148,8 +148,12 @@ int ihg_dp853_finish(ihg_ctx *c, double dt, const double *y,
  * given by its sizes: nlive = the live prefix the stage kernels stream
  * (Ntot - 1 with the GW blocks, 4 N without them; even), ntot = Ntot (the
  * scale factor is y[ntot - 1]).  The block partials and the two sums live
- * in a fixed device array of the library (65538 doubles, 512 KiB, one per
- * GPU, held until the process ends); nothing is allocated or freed.  Work
+ * in a fixed device array of the library (67586 doubles, 528 KiB, one per
+ * GPU, held until the process ends); nothing is allocated or freed.  Up to
+ * 8.4 M element pairs (nlive <= 16777216) its work-groups and the order of
+ * its sums are ihg_dp853_finish's, so the results are the same bit for
+ * bit; beyond, its 32768 work-groups stride, and the sums agree to
+ * rounding.  Work
  * is queued on `stream` (a hipStream_t of the current device, NULL: the
  * null stream), which is synchronised before the sums are returned.  Every
  * call on one GPU shares that GPU's array, so calls that overlap on one GPU

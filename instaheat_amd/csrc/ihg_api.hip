@@ -205,7 +205,7 @@ extern "C" int ihg_destroy(ihg_ctx *c)
     for (hipEvent_t e : c->ev_pool)
         (void)hipEventDestroy(e);
     void *dev[] = {c->tw, c->twf, c->filter, c->W, c->Wc, c->grad3, c->rho,
-                   c->partials, c->dscal, c->dobs, c->gather};
+                   c->partials, c->dscal, c->dobs, c->gather, c->fin_partials};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     for (void *p : c->user_arrays)

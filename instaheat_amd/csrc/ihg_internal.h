@@ -106,6 +106,8 @@ struct ihg_ctx {
     double *tmpf;      // N, Laplacian of psi on output steps (tmp.f)
     double *partials;  // per-block reduction partials
     long npartials;
+    double *fin_partials = nullptr; // k_dp_finish's block partials and the
+    long fin_npartials = 0;         // second level of their sum (on demand)
     double *dscal;     // device scalars: [0] rho_mean, [1] pressure_mean, ...
     double *hscal;     // pinned host mirror of dscal
     double *dobs;      // device observables (packed)

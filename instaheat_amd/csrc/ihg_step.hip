@@ -247,6 +247,31 @@ __global__ __launch_bounds__(RED_WG) void k_dp_finish(
     }
 }
 
+// first level of the sum of many block-partial pairs: work-group g adds the
+// pairs [g * chunk, (g + 1) * chunk) (a fixed partition: deterministic)
+__global__ __launch_bounds__(RED_WG) void k_sum2_mid(
+    const double *__restrict__ part, long nb, long chunk,
+    double *__restrict__ out)
+{
+    __shared__ double sh[8];
+    const long b0 = (long)blockIdx.x * chunk;
+    const long b1 = b0 + chunk < nb ? b0 + chunk : nb;
+    const double2 *p2 = reinterpret_cast<const double2 *>(part);
+    double s0 = 0.0, s1 = 0.0;
+#pragma unroll 4
+    for (long b = b0 + threadIdx.x; b < b1; b += RED_WG) {
+        const double2 v = p2[b];
+        s0 += v.x;
+        s1 += v.y;
+    }
+    s0 = block_reduce<OpSum>(s0, sh, RED_WG);
+    s1 = block_reduce<OpSum>(s1, sh, RED_WG);
+    if (threadIdx.x == 0) {
+        out[2 * blockIdx.x] = s0;
+        out[2 * blockIdx.x + 1] = s1;
+    }
+}
+
 __global__ __launch_bounds__(RED_WG) void k_sum2_final(
     const double *__restrict__ part, int nb, double *__restrict__ out)
 {
@@ -577,15 +602,68 @@ static int finish_inputs(const double *y, const double *const *k,
     return 0;
 }
 
-// work-groups of k_dp_finish: about four pairs per thread, enough
-// work-groups in flight to keep the nine input streams at full rate, at
-// most FINISH_MAX_GROUPS (the kernel strides over the rest)
+// work-groups of the allocation-free finish beyond which its kernel
+// strides over the rest (its fixed array holds that many partials)
 constexpr long FINISH_MAX_GROUPS = 32768;
-static long finish_groups(long nl)
+
+// The context's finish takes one element pair per thread (up to
+// FINISH_WIDE_GROUPS work-groups; 263 K at 256^3): streaming without the
+// grid-stride loop runs at 5.3 against 4.9 TB/s.  Its block partials are
+// summed in two levels, FINISH_MID work-groups over fixed chunks and then
+// one, so the order of the sums is fixed.
+constexpr long FINISH_WIDE_GROUPS = 1L << 20;
+constexpr long FINISH_MID = 1024;
+static long finish_groups_wide(long nl)
 {
-    long nb = (nl / 2 + RED_WG * 4 - 1) / (RED_WG * 4);
-    if (nb > FINISH_MAX_GROUPS) nb = FINISH_MAX_GROUPS;
+    long nb = (nl / 2 + RED_WG - 1) / RED_WG;
+    if (nb > FINISH_WIDE_GROUPS) nb = FINISH_WIDE_GROUPS;
     return nb < 1 ? 1 : nb;
+}
+
+// work-groups of a finish and of the first level of its sum
+struct FinishPlan {
+    long nb, nmid, chunk;
+    long doubles() const { return 2 * nb + 2 * nmid + 2; } // + the sums
+};
+
+static FinishPlan finish_plan(long nb)
+{
+    FinishPlan p;
+    p.nb = nb;
+    p.nmid = nb < FINISH_MID ? 1 : FINISH_MID;
+    p.chunk = (nb + p.nmid - 1) / p.nmid;
+    return p;
+}
+
+// k_dp_finish over nlive with its partials in part[0, 2 nb)
+static int launch_finish(hipStream_t s, const FinishPlan &fp, const double *y,
+                         const FinishIn &in, double *y_new, double dt,
+                         double rtol, double atol, long nlive, long ntot,
+                         double *part)
+{
+    hipLaunchKernelGGL((streaming(nlive) ? k_dp_finish<true>
+                                         : k_dp_finish<false>),
+                       dim3((unsigned)fp.nb), dim3(RED_WG), 0, s, y, in, y_new,
+                       dt, rtol, atol, nlive, ntot, part);
+    IHG_LAUNCHED("k_dp_finish");
+    return 0;
+}
+
+// the two sums of those partials: the first level in part[2 nb, 2 nb +
+// 2 nmid), the sums in the last two doubles of the plan
+static int launch_finish_sums(hipStream_t s, const FinishPlan &fp, double *part)
+{
+    double *mid = part + 2 * fp.nb, *sums = mid + 2 * fp.nmid;
+    if (fp.nmid > 1) {
+        hipLaunchKernelGGL(k_sum2_mid, dim3((unsigned)fp.nmid), dim3(RED_WG),
+                           0, s, part, fp.nb, fp.chunk, mid);
+        IHG_LAUNCHED("k_sum2_mid");
+    }
+    hipLaunchKernelGGL(k_sum2_final, dim3(1), dim3(RED_WG), 0, s,
+                       fp.nmid > 1 ? mid : part,
+                       (int)(fp.nmid > 1 ? fp.nmid : fp.nb), sums);
+    IHG_LAUNCHED("k_sum2_final");
+    return 0;
 }
 
 extern "C" int ihg_dp853_finish(ihg_ctx *c, double dt, const double *y,
@@ -596,21 +674,24 @@ extern "C" int ihg_dp853_finish(ihg_ctx *c, double dt, const double *y,
     FinishIn in{};
     IHG_TRY(finish_inputs(y, k, y_new, in));
     const long nl = live_len(c);
-    const long nb = finish_groups(nl);
-    if (2 * nb + 2 > c->npartials) return set_error("partials buffer too small");
+    const FinishPlan fp = finish_plan(finish_groups_wide(nl));
+    if (fp.doubles() > c->fin_npartials) {
+        // once per context (and size): the first finish allocates
+        IHG_HIP(hipStreamSynchronize(c->stream));
+        if (c->fin_partials) IHG_HIP(hipFree(c->fin_partials));
+        c->fin_partials = nullptr;
+        c->fin_npartials = 0;
+        IHG_HIP(hipMalloc((void **)&c->fin_partials, fp.doubles() * sizeof(double)));
+        c->fin_npartials = fp.doubles();
+    }
+    double *sums_dev = c->fin_partials + fp.doubles() - 2;
     // y and the 8 weighted stages in, y_new out
     IHG_TRY(timed(c, F_DP_FINISH, 10.0 * 8.0 * nl, [&]() {
-        hipLaunchKernelGGL((streaming(nl) ? k_dp_finish<true>
-                                          : k_dp_finish<false>),
-                           dim3((unsigned)nb), dim3(RED_WG), 0,
-                           c->stream, y, in, y_new, dt, rtol, atol, nl,
-                           c->Ntot, c->partials);
-        return hip_check(hipGetLastError(), "k_dp_finish");
+        return launch_finish(c->stream, fp, y, in, y_new, dt, rtol, atol, nl,
+                             c->Ntot, c->fin_partials);
     }));
-    hipLaunchKernelGGL(k_sum2_final, dim3(1), dim3(RED_WG), 0, c->stream,
-                       c->partials, (int)nb, c->dscal + 8);
-    IHG_LAUNCHED("k_sum2_final");
-    IHG_HIP(hipMemcpyAsync(c->hscal + 8, c->dscal + 8, 2 * sizeof(double),
+    IHG_TRY(launch_finish_sums(c->stream, fp, c->fin_partials));
+    IHG_HIP(hipMemcpyAsync(c->hscal + 8, sums_dev, 2 * sizeof(double),
                            hipMemcpyDeviceToHost, c->stream));
     IHG_HIP(hipStreamSynchronize(c->stream));
     sums[0] = c->hscal[8];
@@ -622,8 +703,9 @@ extern "C" int ihg_dp853_finish(ihg_ctx *c, double dt, const double *y,
 // The block partials and the two sums live in `scratch`, one copy per GPU
 // for the life of the process.  Host code never passes `scratch` itself:
 // its device address, looked up once per device, is kept in scratch_dev.
-constexpr long NOALLOC_SUMS = 2 * FINISH_MAX_GROUPS; // offset of the sums
-constexpr long NOALLOC_DOUBLES = NOALLOC_SUMS + 2;
+// the partials of FINISH_MAX_GROUPS work-groups, the first level of their
+// sum and the two sums
+constexpr long NOALLOC_DOUBLES = 2 * FINISH_MAX_GROUPS + 2 * FINISH_MID + 2;
 static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
 
 constexpr int NOALLOC_MAX_DEVICES = 64;
@@ -667,18 +749,18 @@ extern "C" int ihg_dp853_finish_noalloc(void *stream, long nlive, long ntot,
     double *part = nullptr;
     IHG_TRY(scratch_of_current_device(&part));
     hipStream_t s = static_cast<hipStream_t>(stream);
-    const long nb = finish_groups(nlive);
-    // every partial k_sum2_final reads is written by k_dp_finish first, so
-    // the scratch needs no clearing between calls
-    hipLaunchKernelGGL((streaming(nlive) ? k_dp_finish<true>
-                                         : k_dp_finish<false>),
-                       dim3((unsigned)nb), dim3(RED_WG), 0, s,
-                       y, in, y_new, dt, rtol, atol, nlive, ntot, part);
-    IHG_LAUNCHED("k_dp_finish");
-    hipLaunchKernelGGL(k_sum2_final, dim3(1), dim3(RED_WG), 0, s, part,
-                       (int)nb, part + NOALLOC_SUMS);
-    IHG_LAUNCHED("k_sum2_final");
-    IHG_HIP(hipMemcpyAsync(sums, part + NOALLOC_SUMS, 2 * sizeof(double),
+    // the context's work-groups (one element pair per thread) when the
+    // scratch holds their partials -- the same partition of the sums, so the
+    // same values -- and FINISH_MAX_GROUPS striding work-groups beyond
+    const long wide = finish_groups_wide(nlive);
+    const FinishPlan fp =
+        finish_plan(wide <= FINISH_MAX_GROUPS ? wide : FINISH_MAX_GROUPS);
+    // every partial the sums read is written by the kernels before, so the
+    // scratch needs no clearing between calls
+    IHG_TRY(launch_finish(s, fp, y, in, y_new, dt, rtol, atol, nlive, ntot,
+                          part));
+    IHG_TRY(launch_finish_sums(s, fp, part));
+    IHG_HIP(hipMemcpyAsync(sums, part + fp.doubles() - 2, 2 * sizeof(double),
                            hipMemcpyDeviceToHost, s));
     IHG_HIP(hipStreamSynchronize(s));
     return 0;
