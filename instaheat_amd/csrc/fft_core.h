@@ -225,6 +225,21 @@ struct Lds {
 // the wavefront's LDS accesses are performed in order: only the compiler
 // must keep them in order, and the other wavefronts of the work-group run
 // on without waiting (the transposes are wavefront-local exchanges).
+// Work-group barrier of an LDS exchange.  __syncthreads() fences every
+// address space at work-group scope, which waits for all of the wave's
+// outstanding global loads and stores too (s_waitcnt vmcnt(0)): loads issued
+// ahead of a transform, or the stores of the previous one, would be drained
+// at every stage.  The exchanges need only the LDS operations done and
+// ordered, so the fences are on the LDS address space alone (lgkmcnt(0) and
+// s_barrier); a global load still in flight is waited for where its value is
+// used.  No work-group barrier here orders global memory between threads.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 template <bool WAVE>
 __device__ __forceinline__ void line_sync()
 {
@@ -233,8 +248,30 @@ __device__ __forceinline__ void line_sync()
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     } else {
-        __syncthreads();
+        lds_barrier();
     }
+}
+
+// The stage twiddles tw[L, 2L) copied into the LDS table ltw by the calling
+// wavefront, and the pointer through which stages index them as tw[L + i].
+// Twiddles read from LDS are counted apart from the vector-memory loads,
+// which return in order: a twiddle read from global memory waits for every
+// load issued before it, so loads issued ahead of a transform would be
+// drained at its second stage.  Every wavefront of a work-group writes the
+// same values, so no work-group barrier is needed: a wavefront reads its
+// own writes, or identical ones.  ltw must lie at least L values into the
+// work-group's LDS.
+template <int L, class C>
+__device__ __forceinline__ const C *twiddles_to_lds_wave(C *ltw,
+                                                         const C *__restrict__ tw)
+{
+    const int lane = (int)(threadIdx.x % 64);
+#pragma unroll
+    for (int k = 0; k < (L + 63) / 64; ++k)
+        if (lane + 64 * k < L)
+            ltw[lane + 64 * k] = tw[L + lane + 64 * k];
+    line_sync<true>();
+    return ltw - L;
 }
 
 // One Stockham stage of radix R on a line in LDS.  Ns = product of the
@@ -384,6 +421,40 @@ constexpr int plan_stages()
     return 1 + (P::R1 > 1) + (P::R2 > 1) + (P::R3 > 1);
 }
 
+// radix of stage S of a plan, and the product of the radices before it
+template <class P, int S>
+__host__ __device__ constexpr int stage_radix()
+{
+    return S == 0 ? P::R0 : S == 1 ? P::R1 : S == 2 ? P::R2 : P::R3;
+}
+template <class P, int S>
+__host__ __device__ constexpr int stage_span()
+{
+    if constexpr (S == 0)
+        return 1;
+    else
+        return stage_span<P, S - 1>() * stage_radix<P, S - 1>();
+}
+
+// Stages S, S + 1, ... of a line transform on the LDS line `line` (in
+// place), whose threads are lanes of one wavefront (wavefront-local
+// exchanges); the last stage's outputs go to wr(slot, n, value).  On entry
+// the line holds the output of stage S - 1 and the wavefront is
+// synchronised with its writers.
+template <int L, int DIR, class P, int S, class C, class Wr>
+__device__ __forceinline__ void fft_stages_wave(C *line, int j,
+                                                const C *__restrict__ tw,
+                                                Wr &wr)
+{
+    constexpr int NST = plan_stages<P>();
+    if constexpr (S < NST) {
+        NoRead rd;
+        stockham_stage<L, stage_radix<P, S>(), stage_span<P, S>(), DIR, false,
+                       S == NST - 1, P, C, true>(line, line, j, tw, rd, wr);
+        fft_stages_wave<L, DIR, P, S + 1, C>(line, j, tw, wr);
+    }
+}
+
 template <int L, int DIR, bool FIRST_FROM_RD = false, bool LAST_TO_WR = false,
           class P = Plan<L>, class C = double2, class Rd = NoRead,
           class Wr = NoWrite>
@@ -393,19 +464,6 @@ __device__ __forceinline__ void fft_line(C *line, int j,
 {
     fft_line_impl<L, DIR, FIRST_FROM_RD, LAST_TO_WR, P, C, false>(line, line, j,
                                                                   tw, rd, wr);
-}
-
-// A line transform from `rd` to `wr` whose stages alternate between the LDS
-// lines `line` and `line2` (ping-pong): one work-group barrier per stage
-// instead of two.  The last stage reads `line` when the plan has an even
-// number of stages and `line2` otherwise (the first stage writes `line`).
-template <int L, int DIR, class P = Plan<L>, class C = double2, class Rd,
-          class Wr>
-__device__ __forceinline__ void fft_line_pp(C *line, C *line2, int j,
-                                            const C *__restrict__ tw, Rd rd,
-                                            Wr wr)
-{
-    fft_line_impl<L, DIR, true, true, P, C, false>(line, line2, j, tw, rd, wr);
 }
 
 // A line whose threads are lanes of one wavefront, first stage from `rd`,

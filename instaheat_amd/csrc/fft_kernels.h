@@ -245,7 +245,7 @@ __device__ __forceinline__ void x_fanout_body(
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
         if (FWD || q > 0)
-            __syncthreads(); // LDS reads of the previous transform are done
+            lds_barrier(); // LDS reads of the previous transform are done
         double2 *dst = out + q * qstride + base;
         auto rq = [&](int slot, int n) {
             return fac(bk, q, line, n, spec[FWD ? chain_slot<L, P>(slot) : slot]);

@@ -239,7 +239,7 @@ struct FamZC2RRho {
         const double2 *tw;
         Ens e;
     };
-    template <int L> static constexpr int lds() { return Tile<L>::B * Tile<L>::PITCH; }
+    template <int L> static constexpr int lds() { return z_rho_lds<L>(); }
     template <int L>
     __device__ static void run(const Blk &bk, double2 *lds, const Args &a)
     {
@@ -347,7 +347,7 @@ struct FamTT {
     template <int L>
     static constexpr int lds()
     {
-        return 2 * (WG / (L / GroupTTPlan<L>::P::E)) * Lds<L>::PITCH;
+        return tt_lds<L, WG, typename GroupTTPlan<L>::P>();
     }
     template <int L>
     __device__ static void run(const Blk &bk, double2 *lds, const Args &a)

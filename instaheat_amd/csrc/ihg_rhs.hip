@@ -72,7 +72,7 @@ __global__ __launch_bounds__(WG) void k_y_tt_update_h_rk4(
     const double *__restrict__ dscal, LineSet ls,
     const double2 *__restrict__ tw, Ens e, long rw, Rk4Ep ep)
 {
-    __shared__ double2 lds[2 * (WG / (L / P::E)) * Lds<L>::PITCH];
+    __shared__ double2 lds[tt_lds<L, WG, P>()];
     y_tt_update_h_body<L, WG, P, double2, true>(hw_blk(), lds, W, M, f, result,
                                                 N, Next, Ntot, kg, dscal, ls,
                                                 tw, e, rw, ep);

@@ -72,6 +72,19 @@ struct FilterFac {
     }
 };
 
+// LDS of the z c2r + rho pass in complex values: the tile's lines, then the
+// L stage twiddles
+template <int L>
+__host__ __device__ constexpr int z_rho_lines()
+{
+    return Tile<L>::B * Tile<L>::PITCH;
+}
+template <int L>
+__host__ __device__ constexpr int z_rho_lds()
+{
+    return z_rho_lines<L>() + L;
+}
+
 // ---------------------------------------------------------------------------
 // z c2r of the four derivative spectra fused with assemble_gradient_squared
 // and mk_rho_and_p (src/toolbox.c:196-243).  Line 2r holds (d_x, d_y) of row
@@ -102,6 +115,9 @@ __device__ __forceinline__ void z_c2r_rho_body(
     const int p = (int)(line & 1);
     const double2 *A = W + (2 * p) * M + row * MZ;
     const double2 *Bq = W + (2 * p + 1) * M + row * MZ;
+    // stage twiddles from LDS behind the tile's lines, so that the loads of
+    // phi, dphi, psi below stay in flight through the whole transform
+    tw = twiddles_to_lds_wave<L>(lds + z_rho_lines<L>(), tw);
     auto rd = [&](int, int n) {
         const double2 z = herm_merge<L>(A, Bq, n); // in bounds for every lane
         return act ? z : make_double2(0.0, 0.0);
@@ -185,7 +201,7 @@ __global__ __launch_bounds__(FFT_WG) void k_z_c2r_rho(
     double *__restrict__ partials, long nrows, long N, double m2,
     const double2 *__restrict__ tw, Ens e)
 {
-    __shared__ double2 lds[Tile<L>::B * Tile<L>::PITCH];
+    __shared__ double2 lds[z_rho_lds<L>()];
     z_c2r_rho_body<L, G>(hw_blk(), lds, W, M, f, result, grad3, rho, pressure,
                          partials, nrows, N, m2, tw, e);
 }
@@ -426,39 +442,44 @@ __device__ __forceinline__ void tt_coef(R kx, R ky, R kz, R inv_ksq,
     be = zero ? (R)0 : dpq - fp * rowq - fq * rowp + half * fp * kq * s1;
 }
 
-// y transform of source component C (its z and x passes are done) with the
+// y transform of source component K (its z and x passes are done) with the
 // TT accumulation folded into the last Stockham stage; `cur` holds the
-// component's first-stage points (loaded one component ahead).  The stages
-// alternate between the LDS lines my[0] and my[1] (ping-pong: one barrier
-// per stage), and each component starts on the line its predecessor's last
-// stage did not read, so no barrier is needed between components.
+// component's first-stage points (loaded ahead).  Two layouts: the
+// work-group's threads, lanes over the B lines as the loads are (thread j
+// of line `lb`), run the first stage from `cur` into the component's LDS
+// buffer, then one work-group barrier; then each wavefront runs the other
+// stages on its own lines (thread jw of line `lw`) with wavefront-local
+// exchanges.  Components alternate between two LDS buffers: the barrier of
+// component K also ends every wavefront's use of the buffer component K + 1
+// writes.  kx, kz, b1 (|k_z| > eps) and z0 (ix = iz = 0) are those of line
+// lw, inv_ksq the 1/k^2 of its points.
 template <int L, class P, int K, class C>
 __device__ __forceinline__ void tt_component(
-    C *const *my, int j, const C *__restrict__ tw, const C *cur,
-    const KGeom &kg, int ix, int iz, const typename CT<C>::R *inv_ksq, C *o0,
-    C *o1)
+    C *lb, C *lw, int j, int jw, const C *__restrict__ tw, const C *cur,
+    const KGeom &kg, typename CT<C>::R kx, typename CT<C>::R kz, bool b1,
+    bool z0, const typename CT<C>::R *inv_ksq, C *o0, C *o1)
 {
     using R = typename CT<C>::R;
-    const R kx = (R)kg.kxf(ix), kz = (R)kg.kzf(iz);
-    const bool b1 = fabs(kg.kzf(iz)) > DBL_EPSILON; // branch on the FP64 k
-    // an even number of stages ends reading the first line: swap per
-    // component; an odd number ends reading the second: keep the order
-    constexpr int PAR = plan_stages<P>() % 2 == 0 ? K % 2 : 0;
     // Register pressure control (empty asm statements, no instructions):
-    // an opaque copy of the twiddle pointer per component stops the compiler
-    // from loading every stage twiddle once and holding it across all six
+    // an opaque zero offset of the twiddle pointer per component (the
+    // pointer itself stays visibly an LDS one) stops the compiler from
+    // loading every stage twiddle once and holding it across all six
     // transforms, and pinning the accumulators after each component stops it
     // from deferring the sums (and so keeping all six spectra live).
-    const C *twc = tw;
-    asm volatile("" : "+s"(twc));
+    int zo = 0;
+    asm volatile("" : "+s"(zo));
+    const C *twc = tw + zo;
     auto rd = [&](int s, int) { return cur[s]; };
+    NoWrite nw;
+    stockham_stage<L, P::R0, 1, -1, true, false, P, C, false>(lb, lb, j, twc,
+                                                              rd, nw);
     auto acc = [&](int s, int n, C v) {
         // recomputed per component from 1/k^2 alone: an empty asm keeps the
         // compiler from holding every derived coefficient of every point
         // live across the six transforms
         R iv = inv_ksq[s];
         asm volatile("" : "+v"(iv));
-        const bool zero = (ix == 0) && (iz == 0) && (n == 0);
+        const bool zero = z0 && (n == 0);
         R al, be;
         tt_coef<K>(kx, (R)kg.kyf(n), kz, iv, b1, zero, al, be);
         if constexpr (K == 0) {
@@ -469,7 +490,7 @@ __device__ __forceinline__ void tt_component(
             o1[s] = CT<C>::make(o1[s].x + be * v.x, o1[s].y + be * v.y);
         }
     };
-    fft_line_pp<L, -1, P>(my[PAR], my[1 - PAR], j, twc, rd, acc);
+    fft_stages_wave<L, -1, P, 1, C>(lw, jw, twc, acc);
 #pragma unroll
     for (int s = 0; s < P::E; ++s)
         asm volatile("" : "+v"(o0[s].x), "+v"(o0[s].y), "+v"(o1[s].x),
@@ -480,7 +501,7 @@ __device__ __forceinline__ void tt_component(
 // so that the two TT accumulators, 1/k^2 and the transform of every point a
 // thread owns stay in registers.
 template <int L> struct TTPlan;
-template <> struct TTPlan<4>   { using P = PlanT<4, 4>; static constexpr int WG = 256; };
+template <> struct TTPlan<4>   { using P = PlanT<2, 2, 2>; static constexpr int WG = 256; };
 template <> struct TTPlan<8>   { using P = PlanT<4, 4, 2>; static constexpr int WG = 256; };
 template <> struct TTPlan<16>  { using P = PlanT<4, 4, 4>; static constexpr int WG = 256; };
 template <> struct TTPlan<32>  { using P = PlanT<4, 4, 4, 2>; static constexpr int WG = 256; };
@@ -488,6 +509,14 @@ template <> struct TTPlan<64>  { using P = PlanT<4, 4, 4, 4>; static constexpr i
 template <> struct TTPlan<128> { using P = PlanT<4, 4, 4, 4, 2>; static constexpr int WG = 512; };
 template <> struct TTPlan<256> { using P = PlanT<4, 4, 4, 4, 4>; static constexpr int WG = 512; };
 template <> struct TTPlan<512> { using P = PlanT<8, 8, 4, 4, 4>; static constexpr int WG = 512; };
+
+// LDS of the fused pass in complex values: two lines per tile line and the
+// L stage twiddles
+template <int L, int WG, class P>
+__host__ __device__ constexpr int tt_lds()
+{
+    return 2 * (WG / (L / P::E)) * Lds<L>::PITCH + L;
+}
 
 // Last pass of the GW sources: y transforms of the six components (z and x
 // passes done), the TT projection, and update_h (src/toolbox.c:311-346):
@@ -514,38 +543,52 @@ __device__ __forceinline__ void y_tt_update_h_body(
     const long base = ls.base(lc);
     const int ix = (int)(lc / kg.Mz), iz = (int)(lc - (long)ix * kg.Mz);
     const long st = ls.stride;
-    // the two ping-pong lines of this thread's line (lds holds 2 B lines)
-    C *const my[2] = {lds + b * PITCH, lds + (B + b) * PITCH};
+    // The stage twiddles tw[L, 2L) are staged in LDS behind the lines and
+    // read from there: LDS reads are counted apart from the vector-memory
+    // loads, which return in order, so a twiddle read never waits for the
+    // source and h loads issued ahead of it (from global memory it would
+    // wait for all of them).  The first stage, whose barrier makes them
+    // visible, takes no twiddle.
+    {
+        C *const ltw = lds + 2 * B * PITCH;
+        for (int i = tid; i < L; i += WG)
+            ltw[i] = tw[L + i];
+        tw = ltw - L; // indices L + [0, L) of the table
+    }
+    // The wavefront layout of the stages after the first: wavefront w owns
+    // lines w LPW + [0, LPW) of the tile, thread jw of line bw.  (The
+    // wavefront index is uniform: with one line per wavefront its
+    // coordinates and k_x, k_z live in scalar registers.)
+    constexpr int LPW = 64 / T;
+    const int wv = __builtin_amdgcn_readfirstlane(tid / 64);
+    const int bw = wv * LPW + (tid % 64) / T, jw = tid % T;
+    const long linew = (long)tile_x<true>(bk) * B + bw;
+    const long lcw = linew < ls.nlines ? linew : 0;
+    const int ixw = (int)(lcw / kg.Mz), izw = (int)(lcw - (long)ixw * kg.Mz);
+    const R kxw = (R)kg.kxf(ixw), kzw = (R)kg.kzf(izw);
+    const bool b1w = fabs(kg.kzf(izw)) > DBL_EPSILON; // branch on the FP64 k
+    const bool z0w = (ixw == 0) && (izw == 0);
     R inv[E];
 #pragma unroll
     for (int s = 0; s < E; ++s) {
-        const int n = last_point<L, P>(j, s);
-        const bool zero = (ix == 0) && (iz == 0) && (n == 0);
-        inv[s] = zero ? (R)0 : (R)(1.0 / kg.ksq(ix, n, iz));
+        const int n = last_point<L, P>(jw, s);
+        inv[s] = (z0w && n == 0) ? (R)0 : (R)(1.0 / kg.ksq(ixw, n, izw));
     }
-    // first-stage points of component c, in bounds for every lane; the
-    // loads of component c + 1 are issued before the transform of c (the
-    // barriers wait on LDS only), so their latency hides behind it
+    // the two LDS buffers, at this thread's line of each layout
+    C *const lb0 = lds + b * PITCH, *const lb1 = lds + (B + b) * PITCH;
+    C *const lw0 = lds + bw * PITCH, *const lw1 = lds + (B + bw) * PITCH;
+    // first-stage points of component c, in bounds for every lane (an idle
+    // lane loads line 0's points and its results are never stored); the
+    // loads of component c + 1 are issued before the transform of c, so
+    // their latency hides behind it.  The loaded values are used as they
+    // are: a select on them right after the load would wait for it there.
+    // (Loading two components ahead measured no faster.)
     auto load = [&](int c, C *buf) {
         const C *src = W + c * M + base;
 #pragma unroll
-        for (int s = 0; s < E; ++s) {
-            const C v = src[(long)stage1_point<L, P>(j, s) * st];
-            buf[s] = act ? v : CT<C>::make(0, 0);
-        }
+        for (int s = 0; s < E; ++s)
+            buf[s] = src[(long)stage1_point<L, P>(j, s) * st];
     };
-    C o0[E], o1[E], ba[E], bb[E];
-    load(0, ba);
-    load(1, bb);
-    tt_component<L, P, 0>(my, j, tw, ba, kg, ix, iz, inv, o0, o1);
-    load(2, ba);
-    tt_component<L, P, 1>(my, j, tw, bb, kg, ix, iz, inv, o0, o1);
-    load(3, bb);
-    tt_component<L, P, 2>(my, j, tw, ba, kg, ix, iz, inv, o0, o1);
-    load(4, ba);
-    tt_component<L, P, 3>(my, j, tw, bb, kg, ix, iz, inv, o0, o1);
-    load(5, bb);
-    tt_component<L, P, 4>(my, j, tw, ba, kg, ix, iz, inv, o0, o1);
     // update_h's inputs (h and hdot at the points this thread ends with;
     // line 0's points for an idle lane, in bounds): with few points per
     // thread they are loaded before the last transform, so their latency
@@ -565,8 +608,40 @@ __device__ __forceinline__ void y_tt_update_h_body(
             d2v[s] = f2[Ndh2 / 2 + i];
         }
     };
-    if constexpr (EARLY) load_h();
-    tt_component<L, P, 5>(my, j, tw, bb, kg, ix, iz, inv, o0, o1);
+    C o0[E], o1[E], bufs[2][E];
+    load(0, bufs[0]);
+    load(1, bufs[1]);
+    auto step = [&](auto kc) {
+        constexpr int K = decltype(kc)::value;
+        if constexpr (K == 5 && EARLY) load_h();
+        tt_component<L, P, K>(K % 2 ? lb1 : lb0, K % 2 ? lw1 : lw0, j, jw, tw,
+                              bufs[K % 2], kg, kxw, kzw, b1w, z0w, inv, o0,
+                              o1);
+        if constexpr (K + 2 < 6) load(K + 2, bufs[K % 2]);
+    };
+    step(std::integral_constant<int, 0>{});
+    step(std::integral_constant<int, 1>{});
+    step(std::integral_constant<int, 2>{});
+    step(std::integral_constant<int, 3>{});
+    step(std::integral_constant<int, 4>{});
+    step(std::integral_constant<int, 5>{});
+    // the two sums back to the load layout through LDS: o0 to buffer 0 (no
+    // wavefront reads it after component 5's barrier), o1 to this
+    // wavefront's own line of buffer 1 once its last reads of it are done
+#pragma unroll
+    for (int s = 0; s < E; ++s)
+        lw0[Lds<L>::idx(last_point<L, P>(jw, s))] = o0[s];
+    line_sync<true>();
+#pragma unroll
+    for (int s = 0; s < E; ++s)
+        lw1[Lds<L>::idx(last_point<L, P>(jw, s))] = o1[s];
+    lds_barrier();
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+        const int n = Lds<L>::idx(last_point<L, P>(j, s));
+        o0[s] = lb0[n];
+        o1[s] = lb1[n];
+    }
     if constexpr (!EARLY) load_h();
     if (!act)
         return;
@@ -612,7 +687,7 @@ __global__ __launch_bounds__(WG) void k_y_tt_update_h(
     const double *__restrict__ dscal, LineSet ls, const C *__restrict__ tw,
     Ens e, long rw)
 {
-    __shared__ C lds[2 * (WG / (L / P::E)) * Lds<L>::PITCH];
+    __shared__ C lds[tt_lds<L, WG, P>()];
     y_tt_update_h_body<L, WG, P, C>(hw_blk(), lds, W, M, f, result, N, Next,
                                     Ntot, kg, dscal, ls, tw, e, rw);
 }
