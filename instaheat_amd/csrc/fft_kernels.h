@@ -271,7 +271,10 @@ __global__ __launch_bounds__(WG) void k_x_fanout(
 // Line plans of the derivative fan-out.  (At 256 points, 8 values per
 // thread in three stages -- PlanT<8, 8, 8, 4>, 166 VGPRs and 3 waves per
 // SIMD instead of 1 -- measured 209-219 us against 190-194 us for the
-// two-stage 16-value plan, with the XCD-aware tile order on both, so the
+// two-stage 16-value plan, with the XCD-aware tile order on both, and
+// still 198-210 against 172-180 us with the stage twiddles read from LDS
+// (158 VGPRs); the 16-value plan loads its twiddles once for all five
+// transforms, so LDS twiddles gain it nothing either (181-184 us).  The
 // default plans stay.)
 template <int L> struct FanPlan { using P = Plan<L>; static constexpr int WG = FFT_WG; };
 
