@@ -3,8 +3,10 @@ library's fixed device array) against ihg_dp853_finish on the same inputs:
 y_new bit for bit and the two error sums bit for bit (same kernels, same
 work-group count, same reduction order).  Two calls in a row, the second
 with a smaller state, so the second reads what the first left in the shared
-array only where it writes first.  The sums are also checked against numpy's
-restatement of error() (src/dopri853.c:469-486) to rounding."""
+array only where it writes first.  The first (64^3, 4160 work-groups) sums
+its block partials in two levels, the second (8 x 16 x 8) in one.  The sums
+are also checked against numpy's restatement of error()
+(src/dopri853.c:469-486) to rounding."""
 import ctypes
 
 import numpy as np
@@ -75,7 +77,7 @@ def _case(ih, shape, gw, seed, dt):
 
 
 def test_noalloc_finish_equals_context_finish_twice(ih):
-    big = _case(ih, (32, 32, 32), True, 3, 1e-2)
+    big = _case(ih, (64, 64, 64), True, 3, 1e-2)
     small = _case(ih, (8, 16, 8), False, 4, 3e-3)
     assert np.all(big > 0) and np.all(small > 0)
 
