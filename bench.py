@@ -498,9 +498,17 @@ def next_rows(args, wl):
                            "note": "GPU: Gaussian phi, dphi and psi, dpsi from "
                                    "the constraints; host: phi, dphi only"}
     print(f"initial data: {out['initial_data']}", file=sys.stderr, flush=True)
-    # --- warm-up to the error-controlled phase ------------------------------
-    sim.set_max_steps(max(args.warmup - 1, 0))
+    # --- the headline run's pre-roll to t = PRE_ROLL_T and its W warm-up
+    # steps: the error-controlled phase ------------------------------------
+    sim.set_tf(PRE_ROLL_T)
     w = sim.run_dopri853()
+    sim.set_tf(1e9)
+    sim.set_dt(w["dt_next"])
+    if args.warmup > 0:
+        sim.set_max_steps(args.warmup - 1)
+        w = sim.run_dopri853()
+    print(f"next rows: pre-roll and warm-up to t = {w['t']:.4f}",
+          file=sys.stderr, flush=True)
     snap = sim.snapshot()
     nsteps = 8
     sim.set_dt(w["dt_next"])
