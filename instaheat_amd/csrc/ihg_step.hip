@@ -9,6 +9,7 @@
 // divides by Ntot.
 #include <cmath>
 #include <mutex>
+#include <type_traits>
 
 #include "dp853_tableau.h"
 #include "gsl_tableau.h"
@@ -327,6 +328,19 @@ struct GslFinishIn {
     int err_mode;
 };
 
+// y_new and the error ratio of one element from its two weighted sums
+__device__ __forceinline__ double gsl_ratio(const GslFinishIn &f, double yv,
+                                            double sb, double sw, double dt,
+                                            double eps_abs, double eps_rel,
+                                            double &rm)
+{
+    const double yn = yv + dt * sb;
+    const double yerr = f.err_mode ? dt * (sw - sb) : dt * sw;
+    const double r = fabs(yerr) / fabs(eps_rel * fabs(yn) + eps_abs);
+    rm = r > rm ? r : rm;
+    return yn;
+}
+
 __device__ __forceinline__ double gsl_point(const GslFinishIn &f, double yv,
                                             const double *kv, double dt,
                                             double eps_abs, double eps_rel,
@@ -340,13 +354,14 @@ __device__ __forceinline__ double gsl_point(const GslFinishIn &f, double yv,
             sw += f.w[j] * kv[j];
         }
     }
-    const double yn = yv + dt * sb;
-    const double yerr = f.err_mode ? dt * (sw - sb) : dt * sw;
-    const double r = fabs(yerr) / fabs(eps_rel * fabs(yn) + eps_abs);
-    rm = r > rm ? r : rm;
-    return yn;
+    return gsl_ratio(f, yv, sb, sw, dt, eps_abs, eps_rel, rm);
 }
 
+// The final sum and the error ratio with the number of weighted stages NK
+// fixed at compile time (the kernel is instantiated per count), so that
+// every stage load of a pair is issued before the sums need it; one element
+// pair per thread.  The block maxima go to part[0, gridDim.x).
+template <bool NT, int NK>
 __global__ __launch_bounds__(RED_WG) void k_gsl_finish(
     const double *__restrict__ y, GslFinishIn f, double *__restrict__ ynew,
     double dt, double eps_abs, double eps_rel, long nlive, long Ntot,
@@ -360,20 +375,28 @@ __global__ __launch_bounds__(RED_WG) void k_gsl_finish(
     double2 *n2 = reinterpret_cast<double2 *>(ynew);
     for (long i = (long)blockIdx.x * RED_WG + threadIdx.x; i < npairs;
          i += stride) {
-        double kx[GSL_MAX_STAGES], ky[GSL_MAX_STAGES];
+        double2 kv[NK];
 #pragma unroll
-        for (int j = 0; j < GSL_MAX_STAGES; ++j) {
-            if (j < f.nk) {
-                const double2 kj = reinterpret_cast<const double2 *>(f.k[j])[i];
-                kx[j] = kj.x;
-                ky[j] = kj.y;
-            }
-        }
-        const double2 yv = y2[i];
+        for (int j = 0; j < NK; ++j)
+            kv[j] = ld2<NT>(reinterpret_cast<const double2 *>(f.k[j]) + i);
+        const double2 yv = ld2<NT>(y2 + i);
         double2 yn;
-        yn.x = gsl_point(f, yv.x, kx, dt, eps_abs, eps_rel, rm);
-        yn.y = gsl_point(f, yv.y, ky, dt, eps_abs, eps_rel, rm);
-        n2[i] = yn;
+        double sb = 0.0, sw = 0.0;
+#pragma unroll
+        for (int j = 0; j < NK; ++j) {
+            sb += f.b[j] * kv[j].x;
+            sw += f.w[j] * kv[j].x;
+        }
+        yn.x = gsl_ratio(f, yv.x, sb, sw, dt, eps_abs, eps_rel, rm);
+        sb = 0.0;
+        sw = 0.0;
+#pragma unroll
+        for (int j = 0; j < NK; ++j) {
+            sb += f.b[j] * kv[j].y;
+            sw += f.w[j] * kv[j].y;
+        }
+        yn.y = gsl_ratio(f, yv.y, sb, sw, dt, eps_abs, eps_rel, rm);
+        st2<NT>(n2 + i, yn);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         const long i = Ntot - 1;
@@ -384,6 +407,22 @@ __global__ __launch_bounds__(RED_WG) void k_gsl_finish(
     }
     rm = block_reduce<OpMax>(rm, sh, RED_WG);
     if (threadIdx.x == 0) part[blockIdx.x] = rm;
+}
+
+// first level of the maximum of many block maxima: work-group g takes
+// part[g * chunk, (g + 1) * chunk)
+__global__ __launch_bounds__(RED_WG) void k_max_mid(
+    const double *__restrict__ part, long nb, long chunk,
+    double *__restrict__ out)
+{
+    __shared__ double sh[8];
+    const long b0 = (long)blockIdx.x * chunk;
+    const long b1 = b0 + chunk < nb ? b0 + chunk : nb;
+    double m = 0.0;
+    for (long b = b0 + threadIdx.x; b < b1; b += RED_WG)
+        m = part[b] > m ? part[b] : m;
+    m = block_reduce<OpMax>(m, sh, RED_WG);
+    if (threadIdx.x == 0) out[blockIdx.x] = m;
 }
 
 __global__ __launch_bounds__(RED_WG) void k_max_final(
@@ -666,6 +705,20 @@ static int launch_finish_sums(hipStream_t s, const FinishPlan &fp, double *part)
     return 0;
 }
 
+// the context's finish partials hold at least n doubles (allocated by the
+// first finish that needs them, once per context and size)
+static int ensure_fin_partials(ihg_ctx *c, long n)
+{
+    if (n <= c->fin_npartials) return 0;
+    IHG_HIP(hipStreamSynchronize(c->stream));
+    if (c->fin_partials) IHG_HIP(hipFree(c->fin_partials));
+    c->fin_partials = nullptr;
+    c->fin_npartials = 0;
+    IHG_HIP(hipMalloc((void **)&c->fin_partials, n * sizeof(double)));
+    c->fin_npartials = n;
+    return 0;
+}
+
 extern "C" int ihg_dp853_finish(ihg_ctx *c, double dt, const double *y,
                                 const double *const *k, double *y_new,
                                 double rtol, double atol, double sums[2])
@@ -675,15 +728,7 @@ extern "C" int ihg_dp853_finish(ihg_ctx *c, double dt, const double *y,
     IHG_TRY(finish_inputs(y, k, y_new, in));
     const long nl = live_len(c);
     const FinishPlan fp = finish_plan(finish_groups_wide(nl));
-    if (fp.doubles() > c->fin_npartials) {
-        // once per context (and size): the first finish allocates
-        IHG_HIP(hipStreamSynchronize(c->stream));
-        if (c->fin_partials) IHG_HIP(hipFree(c->fin_partials));
-        c->fin_partials = nullptr;
-        c->fin_npartials = 0;
-        IHG_HIP(hipMalloc((void **)&c->fin_partials, fp.doubles() * sizeof(double)));
-        c->fin_npartials = fp.doubles();
-    }
+    IHG_TRY(ensure_fin_partials(c, fp.doubles()));
     double *sums_dev = c->fin_partials + fp.doubles() - 2;
     // y and the 8 weighted stages in, y_new out
     IHG_TRY(timed(c, F_DP_FINISH, 10.0 * 8.0 * nl, [&]() {
@@ -861,6 +906,30 @@ extern "C" int ihg_gsl_stage(ihg_ctx *c, int method, int s, double dt,
     });
 }
 
+// f(std::integral_constant<int, n>) for the weighted-stage counts of the
+// GSL steppers' final sums, n = 1..GSL_MAX_STAGES
+static_assert(GSL_MAX_STAGES == 13, "one case per stage count");
+template <class F>
+static int dispatch_gsl_stages(int n, F &&f)
+{
+    switch (n) {
+    case 1: return f(std::integral_constant<int, 1>{});
+    case 2: return f(std::integral_constant<int, 2>{});
+    case 3: return f(std::integral_constant<int, 3>{});
+    case 4: return f(std::integral_constant<int, 4>{});
+    case 5: return f(std::integral_constant<int, 5>{});
+    case 6: return f(std::integral_constant<int, 6>{});
+    case 7: return f(std::integral_constant<int, 7>{});
+    case 8: return f(std::integral_constant<int, 8>{});
+    case 9: return f(std::integral_constant<int, 9>{});
+    case 10: return f(std::integral_constant<int, 10>{});
+    case 11: return f(std::integral_constant<int, 11>{});
+    case 12: return f(std::integral_constant<int, 12>{});
+    case 13: return f(std::integral_constant<int, 13>{});
+    default: return set_error("no GSL final sum of %d stages", n);
+    }
+}
+
 extern "C" int ihg_gsl_finish(ihg_ctx *c, int method, double dt,
                               const double *y, const double *const *k,
                               double *y_new, double eps_abs, double eps_rel,
@@ -884,21 +953,34 @@ extern "C" int ihg_gsl_finish(ihg_ctx *c, int method, double dt,
     in.nk = n;
     in.err_mode = t->err_mode;
     const long nl = live_len(c);
-    long nb = (nl / 2 + RED_WG * 4 - 1) / (RED_WG * 4);
-    if (nb > 32768) nb = 32768;
-    if (nb < 1) nb = 1;
-    if (nb + 2 > c->npartials) return set_error("partials buffer too small");
+    // one element pair per thread; the block maxima in part[0, nb), their
+    // first level in part[nb, nb + nmid), the maximum at part[nb + nmid]
+    const FinishPlan fp = finish_plan(finish_groups_wide(nl));
+    IHG_TRY(ensure_fin_partials(c, fp.nb + fp.nmid + 1));
+    double *part = c->fin_partials, *mid = part + fp.nb,
+           *dst = mid + fp.nmid;
     // y and the weighted stages in, y_new out
     IHG_TRY(timed(c, F_GSL_FINISH, (n + 2.0) * 8.0 * nl, [&]() {
-        hipLaunchKernelGGL(k_gsl_finish, dim3((unsigned)nb), dim3(RED_WG), 0,
-                           c->stream, y, in, y_new, dt, eps_abs, eps_rel, nl,
-                           c->Ntot, c->partials);
-        return hip_check(hipGetLastError(), "k_gsl_finish");
+        return dispatch_gsl_stages(n, [&](auto nk) {
+            constexpr int NK = decltype(nk)::value;
+            hipLaunchKernelGGL((streaming(nl) ? k_gsl_finish<true, NK>
+                                              : k_gsl_finish<false, NK>),
+                               dim3((unsigned)fp.nb), dim3(RED_WG), 0,
+                               c->stream, y, in, y_new, dt, eps_abs, eps_rel,
+                               nl, c->Ntot, part);
+            return hip_check(hipGetLastError(), "k_gsl_finish");
+        });
     }));
+    if (fp.nmid > 1) {
+        hipLaunchKernelGGL(k_max_mid, dim3((unsigned)fp.nmid), dim3(RED_WG), 0,
+                           c->stream, part, fp.nb, fp.chunk, mid);
+        IHG_LAUNCHED("k_max_mid");
+    }
     hipLaunchKernelGGL(k_max_final, dim3(1), dim3(RED_WG), 0, c->stream,
-                       c->partials, (int)nb, c->dscal + 8);
+                       fp.nmid > 1 ? mid : part,
+                       (int)(fp.nmid > 1 ? fp.nmid : fp.nb), dst);
     IHG_LAUNCHED("k_max_final");
-    IHG_HIP(hipMemcpyAsync(c->hscal + 8, c->dscal + 8, sizeof(double),
+    IHG_HIP(hipMemcpyAsync(c->hscal + 8, dst, sizeof(double),
                            hipMemcpyDeviceToHost, c->stream));
     IHG_HIP(hipStreamSynchronize(c->stream));
     // the control's running maximum starts at DBL_MIN
